@@ -1,0 +1,276 @@
+// C ABI over the C++ classes: opaque handles, exceptions -> status codes.
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "host_internal.hpp"
+#include "pbf/pbf_solver.hpp"
+#include "pbf_amd.h"
+#include "pbf_launch.hpp"
+
+struct pbf_grid {
+    pbf::SpatialGrid grid;
+};
+struct pbf_collision {
+    pbf::CollisionSystem system;
+};
+struct pbf_solver {
+    pbf::PBFSolver solver;
+};
+
+namespace {
+
+// Runs f and maps the reference's exception types onto status codes.
+template <typename F>
+pbf_status guarded(F&& f) {
+    try {
+        f();
+        return PBF_OK;
+    } catch (const pbf::collision_nonconvergence& e) {
+        return pbf_fail(PBF_ERR_NONCONVERGENCE, e.what());
+    } catch (const std::overflow_error& e) {
+        return pbf_fail(PBF_ERR_OVERFLOW, e.what());
+    } catch (const std::invalid_argument& e) {
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, e.what());
+    } catch (const std::out_of_range& e) {
+        return pbf_fail(PBF_ERR_OUT_OF_RANGE, e.what());
+    } catch (const std::length_error& e) {
+        return pbf_fail(PBF_ERR_LENGTH, e.what());
+    } catch (const std::logic_error& e) {
+        return pbf_fail(PBF_ERR_LOGIC, e.what());
+    } catch (const std::bad_alloc& e) {
+        return pbf_fail(PBF_ERR_DEVICE, e.what());
+    } catch (const std::exception& e) {
+        return pbf_fail(PBF_ERR_DEVICE, e.what());
+    } catch (...) {
+        return pbf_fail(PBF_ERR_DEVICE, "unknown error");
+    }
+}
+
+#define PBF_REQUIRE(ptr)                                                         \
+    do {                                                                         \
+        if ((ptr) == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null handle"); \
+    } while (0)
+
+template <typename T>
+std::vector<T> to_vector(const T* data, size_t count) {
+    if (count != 0 && data == nullptr) throw std::invalid_argument("null collider array");
+    return std::vector<T>(data, data + count);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- grid
+pbf_status pbf_grid_create(pbf_grid** grid) {
+    PBF_REQUIRE(grid);
+    return guarded([&] { *grid = new pbf_grid(); });
+}
+
+pbf_status pbf_grid_destroy(pbf_grid* grid) {
+    delete grid;
+    return PBF_OK;
+}
+
+pbf_status pbf_grid_initialize(pbf_grid* grid, size_t maxParticles, pbf_float3 minBounds,
+                               pbf_float3 maxBounds, float cellSize) {
+    PBF_REQUIRE(grid);
+    return guarded([&] { grid->grid.initialize(maxParticles, minBounds, maxBounds, cellSize); });
+}
+
+pbf_status pbf_grid_build(pbf_grid* grid, const pbf_float4* d_predicted, size_t n) {
+    PBF_REQUIRE(grid);
+    return guarded([&] { grid->grid.build(d_predicted, n); });
+}
+
+const uint32_t* pbf_grid_sorted_keys(const pbf_grid* g) { return g ? g->grid.sortedKeys() : nullptr; }
+const uint32_t* pbf_grid_sorted_indices(const pbf_grid* g) {
+    return g ? g->grid.sortedIndices() : nullptr;
+}
+const int* pbf_grid_cell_start(const pbf_grid* g) { return g ? g->grid.cellStart() : nullptr; }
+const int* pbf_grid_cell_end(const pbf_grid* g) { return g ? g->grid.cellEnd() : nullptr; }
+
+pbf_status pbf_grid_info(const pbf_grid* g, pbf_int3* gridSize, size_t* numCells,
+                         size_t* maxParticles, float* cellSize, pbf_float3* minBounds,
+                         pbf_float3* maxBounds) {
+    PBF_REQUIRE(g);
+    if (gridSize) *gridSize = g->grid.gridSize();
+    if (numCells) *numCells = g->grid.numCells();
+    if (maxParticles) *maxParticles = g->grid.maxParticles();
+    if (cellSize) *cellSize = g->grid.cellSize();
+    if (minBounds) *minBounds = g->grid.minBounds();
+    if (maxBounds) *maxBounds = g->grid.maxBounds();
+    return PBF_OK;
+}
+
+// ---------------------------------------------------------------- collisions
+pbf_status pbf_collision_create(pbf_collision** cs) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { *cs = new pbf_collision(); });
+}
+
+pbf_status pbf_collision_destroy(pbf_collision* cs) {
+    delete cs;
+    return PBF_OK;
+}
+
+pbf_status pbf_collision_set_container(pbf_collision* cs, pbf_container c) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.setContainer(c); });
+}
+
+pbf_status pbf_collision_set_spheres(pbf_collision* cs, const pbf_sphere* s, size_t n) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.setSpheres(to_vector(s, n)); });
+}
+pbf_status pbf_collision_clear_spheres(pbf_collision* cs) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.clearSpheres(); });
+}
+pbf_status pbf_collision_set_boxes(pbf_collision* cs, const pbf_box* b, size_t n) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.setBoxes(to_vector(b, n)); });
+}
+pbf_status pbf_collision_clear_boxes(pbf_collision* cs) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.clearBoxes(); });
+}
+pbf_status pbf_collision_set_planes(pbf_collision* cs, const pbf_plane* p, size_t n) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.setPlanes(to_vector(p, n)); });
+}
+pbf_status pbf_collision_clear_planes(pbf_collision* cs) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.clearPlanes(); });
+}
+
+pbf_status pbf_collision_solve(pbf_collision* cs, pbf_float4* d_predicted, size_t n, float r) {
+    PBF_REQUIRE(cs);
+    return guarded([&] { cs->system.solve(d_predicted, n, r); });
+}
+
+pbf_status pbf_collision_resolve_velocities(pbf_collision* cs, const pbf_float4* d_position,
+                                            const pbf_float4* d_incoming, pbf_float4* d_velocity,
+                                            size_t n, float r, float e, float f) {
+    PBF_REQUIRE(cs);
+    return guarded([&] {
+        cs->system.resolveVelocities(d_position, d_incoming, d_velocity, n, r, e, f);
+        pbf::hip_check(hipDeviceSynchronize());
+    });
+}
+
+// ---------------------------------------------------------------- solver
+pbf_status pbf_solver_create(pbf_solver** solver) {
+    PBF_REQUIRE(solver);
+    return guarded([&] { *solver = new pbf_solver(); });
+}
+
+pbf_status pbf_solver_destroy(pbf_solver* solver) {
+    delete solver;
+    return PBF_OK;
+}
+
+pbf_status pbf_solver_initialize(pbf_solver* s, size_t maxParticles, pbf_float3 minBounds,
+                                 pbf_float3 maxBounds, const pbf_sim_params* params) {
+    PBF_REQUIRE(s);
+    if (params == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null params");
+    return guarded([&] { s->solver.initialize(maxParticles, minBounds, maxBounds, *params); });
+}
+
+pbf_status pbf_solver_set_particles(pbf_solver* s, const pbf_float4* pos, const pbf_float4* vel,
+                                    size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setParticles(pos, vel, n); });
+}
+
+pbf_status pbf_solver_set_particles_device(pbf_solver* s, const pbf_float4* pos,
+                                           const pbf_float4* vel, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setParticlesDevice(pos, vel, n); });
+}
+
+pbf_status pbf_solver_set_spheres(pbf_solver* s, const pbf_sphere* sp, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setSpheres(to_vector(sp, n)); });
+}
+pbf_status pbf_solver_set_boxes(pbf_solver* s, const pbf_box* b, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setBoxes(to_vector(b, n)); });
+}
+pbf_status pbf_solver_set_planes(pbf_solver* s, const pbf_plane* p, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setPlanes(to_vector(p, n)); });
+}
+
+pbf_status pbf_solver_step(pbf_solver* s) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.step(); });
+}
+
+pbf_status pbf_solver_run_one(pbf_solver* s) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.run(); });
+}
+
+pbf_status pbf_solver_run(pbf_solver* s, size_t frames) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.run(frames); });
+}
+
+pbf_status pbf_solver_copy_positions_to_host(pbf_solver* s, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.copyPositionsToHost(out, n); });
+}
+pbf_status pbf_solver_copy_velocities_to_host(pbf_solver* s, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.copyVelocitiesToHost(out, n); });
+}
+pbf_status pbf_solver_copy_positions_to_device(pbf_solver* s, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.copyPositionsToDevice(out, n); });
+}
+pbf_status pbf_solver_copy_velocities_to_device(pbf_solver* s, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.copyVelocitiesToDevice(out, n); });
+}
+
+pbf_status pbf_solver_counts(const pbf_solver* s, size_t* count, size_t* maxParticles) {
+    PBF_REQUIRE(s);
+    if (count) *count = s->solver.particleCount();
+    if (maxParticles) *maxParticles = s->solver.maxParticles();
+    return PBF_OK;
+}
+
+void* pbf_solver_stream(pbf_solver* s) { return s ? s->solver.stream() : nullptr; }
+
+pbf_status pbf_solver_set_profiling(pbf_solver* s, int enabled) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setProfiling(enabled != 0); });
+}
+
+pbf_status pbf_solver_kernel_times(pbf_solver* s, char* names, size_t namesBytes, double* ms,
+                                   uint64_t* launches, size_t capacity, size_t* kernelCount) {
+    PBF_REQUIRE(s);
+    return guarded([&] {
+        const auto times = s->solver.kernelTimes();
+        if (kernelCount) *kernelCount = times.size();
+        std::string joined;
+        for (size_t i = 0; i < times.size(); ++i) {
+            if (i < capacity) {
+                if (ms) ms[i] = times[i].milliseconds;
+                if (launches) launches[i] = times[i].launches;
+            }
+            joined += times[i].name;
+            joined += '\n';
+        }
+        if (names != nullptr && namesBytes > 0) {
+            const size_t len = joined.size() < namesBytes - 1 ? joined.size() : namesBytes - 1;
+            std::memcpy(names, joined.data(), len);
+            names[len] = '\0';
+        }
+    });
+}
+
+}  // extern "C"

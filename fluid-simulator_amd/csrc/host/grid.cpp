@@ -1,0 +1,123 @@
+// SpatialGrid and the grid core the solver shares.
+//
+// Reference: src/pbf/spatial_grid.cu:10-268.
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+
+#include "host_internal.hpp"
+#include "pbf_internal.hpp"
+
+namespace pbf {
+
+void hip_check(hipError_t error) {
+    if (error != hipSuccess) throw std::runtime_error(hipGetErrorString(error));
+}
+
+GridCore GridCore::create(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
+                          float cellSize) {
+    // Validation order and exception types of spatial_grid.cu:11-27.
+    if (maxParticles == 0) throw std::invalid_argument("maxParticles must be greater than zero");
+    if (!std::isfinite(cellSize) || cellSize <= 0.0f)
+        throw std::invalid_argument("cellSize must be greater than zero");
+    if (!std::isfinite(minBounds.x) || !std::isfinite(minBounds.y) ||
+        !std::isfinite(minBounds.z) || !std::isfinite(maxBounds.x) ||
+        !std::isfinite(maxBounds.y) || !std::isfinite(maxBounds.z) ||
+        maxBounds.x <= minBounds.x || maxBounds.y <= minBounds.y || maxBounds.z <= minBounds.z) {
+        throw std::invalid_argument("Invalid spatial grid bounds");
+    }
+    if (maxParticles > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::length_error("Particle count exceeds grid range index capacity");
+
+    GridCore next;
+    next.maxParticles = maxParticles;
+    next.minBounds = minBounds;
+    next.maxBounds = maxBounds;
+    next.cellSize = cellSize;
+
+    // ceil((max - min) / cellSize) in double (spatial_grid.cu:37-53).
+    const auto axis = [cellSize](float lo, float hi) {
+        const double count = std::ceil((static_cast<double>(hi) - static_cast<double>(lo)) /
+                                       static_cast<double>(cellSize));
+        if (!std::isfinite(count) || count < 1.0 ||
+            count > static_cast<double>(std::numeric_limits<int>::max()))
+            throw std::length_error("Spatial grid axis exceeds integer capacity");
+        return static_cast<int>(count);
+    };
+    next.dims.x = axis(minBounds.x, maxBounds.x);
+    next.dims.y = axis(minBounds.y, maxBounds.y);
+    next.dims.z = axis(minBounds.z, maxBounds.z);
+
+    const std::size_t maxCells = static_cast<std::size_t>(std::numeric_limits<std::uint32_t>::max());
+    const std::size_t cx = static_cast<std::size_t>(next.dims.x);
+    const std::size_t cy = static_cast<std::size_t>(next.dims.y);
+    const std::size_t cz = static_cast<std::size_t>(next.dims.z);
+    if (cx > maxCells / cy || cx * cy > maxCells / cz)
+        throw std::length_error("Spatial grid exceeds 32-bit key capacity");
+    next.numCells = cx * cy * cz;
+    next.keyBits = pbfint::radix_key_bits(next.numCells);
+
+    next.keys.allocate(maxParticles);
+    next.indices.allocate(maxParticles);
+    next.sortedKeys.allocate(maxParticles);
+    next.sortedIndices.allocate(maxParticles);
+    next.cellStart.allocate(next.numCells);
+    next.cellEnd.allocate(next.numCells);
+    next.scratch.allocate(pbfint::radix_scratch_words(maxParticles));
+    return next;
+}
+
+void GridCore::sortAndRanges(std::size_t n, hipStream_t stream) {
+    launch_check(pbfint::radix_sort_pairs(keys.data(), nullptr, sortedKeys.data(),
+                                          sortedIndices.data(), keys.data(), indices.data(), n,
+                                          keyBits, scratch.data(), stream));
+    launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
+    launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,
+                                            stream));
+}
+
+SpatialGrid::SpatialGrid() : state_(new State()) {}
+SpatialGrid::~SpatialGrid() = default;
+SpatialGrid::SpatialGrid(SpatialGrid&&) noexcept = default;
+SpatialGrid& SpatialGrid::operator=(SpatialGrid&&) noexcept = default;
+
+void SpatialGrid::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
+                             float cellSize) {
+    GridCore next = GridCore::create(maxParticles, minBounds, maxBounds, cellSize);
+    state_->core = std::move(next);
+}
+
+void SpatialGrid::build(const float4* predicted, std::size_t n) {
+    GridCore& g = state_->core;
+    if (g.maxParticles == 0) throw std::logic_error("SpatialGrid must be initialized before building");
+    if (predicted == nullptr && n != 0)
+        throw std::invalid_argument("predictedPositions must not be null");
+    if (n > g.maxParticles)
+        throw std::out_of_range("Particle count exceeds maximum number of particles");
+    hipStream_t stream = nullptr;
+    if (n == 0) {
+        // An empty build still clears the ranges of a previous build.
+        launch_check(pbfint::launch_fill_cells(g.cellStart.data(), g.cellEnd.data(), g.numCells,
+                                               stream));
+        hip_check(hipDeviceSynchronize());
+        return;
+    }
+    launch_check(pbfint::launch_keys(predicted, g.keys.data(), n, g.minBounds, g.cellSize, g.dims,
+                                     stream));
+    g.sortAndRanges(n, stream);
+    // Surface asynchronous failures from this call (spatial_grid.cu:220-225).
+    hip_check(hipDeviceSynchronize());
+}
+
+const std::uint32_t* SpatialGrid::sortedKeys() const { return state_->core.sortedKeys.data(); }
+const std::uint32_t* SpatialGrid::sortedIndices() const { return state_->core.sortedIndices.data(); }
+const int* SpatialGrid::cellStart() const { return state_->core.cellStart.data(); }
+const int* SpatialGrid::cellEnd() const { return state_->core.cellEnd.data(); }
+int3 SpatialGrid::gridSize() const { return state_->core.dims; }
+std::size_t SpatialGrid::numCells() const { return state_->core.numCells; }
+std::size_t SpatialGrid::maxParticles() const { return state_->core.maxParticles; }
+float SpatialGrid::cellSize() const { return state_->core.cellSize; }
+float3 SpatialGrid::minBounds() const { return state_->core.minBounds; }
+float3 SpatialGrid::maxBounds() const { return state_->core.maxBounds; }
+
+}  // namespace pbf

@@ -1,0 +1,77 @@
+// Library-internal host state shared by SpatialGrid, CollisionSystem and
+// PBFSolver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pbf/collision_system.hpp"
+#include "pbf/device_buffer.hpp"
+#include "pbf/spatial_grid.hpp"
+#include "pbf/types.hpp"
+#include "pbf_colliders.hpp"
+
+namespace pbf {
+
+// Throws the reference exception for a failed launch or HIP call.
+inline void launch_check(pbf_status status) { check(status); }
+void hip_check(hipError_t error);
+
+// Grid configuration and buffers (SpatialGrid::initialize, spatial_grid.cu:10-97).
+struct GridCore {
+    float3 minBounds{};
+    float3 maxBounds{};
+    int3 dims{};
+    std::size_t numCells = 0;
+    std::size_t maxParticles = 0;
+    float cellSize = 0.0f;
+    int keyBits = 0;
+
+    DeviceBuffer<std::uint32_t> keys;           // unsorted keys (also a sort ping-pong buffer)
+    DeviceBuffer<std::uint32_t> indices;        // sort ping-pong buffer for values
+    DeviceBuffer<std::uint32_t> sortedKeys;
+    DeviceBuffer<std::uint32_t> sortedIndices;
+    DeviceBuffer<int> cellStart;
+    DeviceBuffer<int> cellEnd;
+    DeviceBuffer<std::uint32_t> scratch;        // radix histograms and scan partials
+
+    // Validates like the reference and allocates; throws on failure.
+    static GridCore create(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
+                           float cellSize);
+
+    // Stable sort of `keys` (slot values implicit) and the cell ranges.
+    void sortAndRanges(std::size_t particleCount, hipStream_t stream);
+};
+
+struct SpatialGrid::State {
+    GridCore core;
+};
+
+struct CollisionSystem::State {
+    DeviceBuffer<Container> container;
+    DeviceBuffer<std::uint32_t> status;  // fixed-point status word
+    Container hostContainer{};
+    bool hasContainer = false;
+    DeviceBuffer<SphereCollider> spheres;
+    DeviceBuffer<BoxCollider> boxes;
+    DeviceBuffer<PlaneCollider> planes;
+    std::size_t sphereCount = 0;
+    std::size_t boxCount = 0;
+    std::size_t planeCount = 0;
+
+    pbfdev::Colliders view() const {
+        return pbfdev::Colliders{container.data(),
+                                 spheres.data(),
+                                 boxes.data(),
+                                 planes.data(),
+                                 static_cast<int>(sphereCount),
+                                 static_cast<int>(boxCount),
+                                 static_cast<int>(planeCount)};
+    }
+};
+
+}  // namespace pbf

@@ -1,0 +1,521 @@
+// PBFSolver: owns the working set in HBM and enqueues the fused step.
+//
+// Reference: src/pbf/pbf_solver.cu:1-379.
+//
+// The working set stays in cell order between steps, as in the reference;
+// stable ids map it back to API order on copy-out.  Per substep the device
+// runs (reference stage -> kernel here):
+//   integrate + v_in snapshot + first cell keys      -> k_predict_key
+//   per iteration: grid build, reorder, neighbours   -> radix sort, ranges,
+//                                                       k_gather, k_find_neighbors
+//                  lambda                             -> k_lambda_list
+//                  delta-p, apply, collision solve    -> k_delta_apply_collide
+//   post-solve rebuild (if XSPH or vorticity)         -> as above
+//   velocity update + XSPH                            -> k_update_xsph
+//   vorticity                                         -> k_vorticity_list
+//   confinement + velocity contacts                   -> k_confine_resolve
+// Neighbour overflow and collision non-convergence set bits of a device
+// status word; every later kernel that changes the state returns at once,
+// and the host reads the word once when the step (or run) finishes.
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "host_internal.hpp"
+#include "pbf/pbf_solver.hpp"
+#include "pbf_internal.hpp"
+
+namespace pbf {
+namespace {
+
+constexpr int kBlockSize = 256;     // pbf_solver.cu:11
+constexpr int kMaxNeighbors = 256;  // pbf_solver.cu:12
+
+// HIP-event timing of named launches on one stream.
+class Profiler {
+public:
+    ~Profiler() {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        for (const Pending& p : pending_) {
+            (void)hipEventDestroy(p.begin);
+            (void)hipEventDestroy(p.end);
+        }
+    }
+
+    bool enabled = false;
+
+    void begin(const char* name, hipStream_t stream) {
+        if (!enabled) return;
+        Pending p;
+        p.id = id_of(name);
+        p.begin = take();
+        p.end = take();
+        hip_check(hipEventRecord(p.begin, stream));
+        pending_.push_back(p);
+    }
+
+    void end(hipStream_t stream) {
+        if (!enabled || pending_.empty()) return;
+        hip_check(hipEventRecord(pending_.back().end, stream));
+    }
+
+    // Call after the stream has been synchronised.
+    void collect() {
+        for (const Pending& p : pending_) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, p.begin, p.end) == hipSuccess) {
+                ms_[p.id] += ms;
+                launches_[p.id] += 1;
+            }
+            pool_.push_back(p.begin);
+            pool_.push_back(p.end);
+        }
+        pending_.clear();
+    }
+
+    std::vector<PBFSolver::KernelTime> times() const {
+        std::vector<PBFSolver::KernelTime> out;
+        for (std::size_t i = 0; i < names_.size(); ++i)
+            out.push_back({names_[i], ms_[i], launches_[i]});
+        return out;
+    }
+
+    void reset() {
+        for (auto& m : ms_) m = 0.0;
+        for (auto& l : launches_) l = 0;
+    }
+
+private:
+    struct Pending {
+        int id;
+        hipEvent_t begin;
+        hipEvent_t end;
+    };
+
+    int id_of(const char* name) {
+        for (std::size_t i = 0; i < names_.size(); ++i)
+            if (names_[i] == name) return static_cast<int>(i);
+        names_.emplace_back(name);
+        ms_.push_back(0.0);
+        launches_.push_back(0);
+        return static_cast<int>(names_.size() - 1);
+    }
+
+    hipEvent_t take() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e));
+        return e;
+    }
+
+    std::vector<std::string> names_;
+    std::vector<double> ms_;
+    std::vector<std::uint64_t> launches_;
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+};
+
+class Scope {
+public:
+    Scope(Profiler& p, const char* name, hipStream_t s) : p_(p), s_(s) { p_.begin(name, s_); }
+    ~Scope() { p_.end(s_); }
+
+private:
+    Profiler& p_;
+    hipStream_t s_;
+};
+
+bool finite4(const float4& v) {
+    return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z);
+}
+
+}  // namespace
+
+struct PBFSolver::State {
+    std::size_t maxParticles = 0;
+    std::size_t count = 0;
+    std::size_t stride = 0;
+    SimulationParams params{};
+    GridCore grid;
+    CollisionSystem collision;
+
+    DeviceBuffer<float4> x, xa, p, pa, v, va, vin, vina, omega;
+    DeviceBuffer<std::uint32_t> id, ida, nbr, status;
+    DeviceBuffer<int> cnt;
+    DeviceBuffer<float> lambda;
+
+    hipStream_t stream = nullptr;
+    Profiler prof;
+
+    ~State() {
+        if (stream != nullptr) (void)hipStreamDestroy(stream);
+    }
+
+    void ensure_stream() {
+        if (stream == nullptr) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    }
+
+    // Grid build of the current p*, then the permutation of the working set
+    // into cell order (SpatialGrid::build + reorderWorkingSet,
+    // pbf_solver.cu:154-172, spatial_grid.cu:139-226).  `keysReady` skips
+    // the key pass when the predict kernel already wrote the keys.
+    void build_and_reorder(bool keysReady) {
+        const std::size_t n = count;
+        if (!keysReady) {
+            Scope sc(prof, "keys", stream);
+            launch_check(pbfint::launch_keys(p.data(), grid.keys.data(), n, grid.minBounds,
+                                             grid.cellSize, grid.dims, stream));
+        }
+        {
+            Scope sc(prof, "radix_sort", stream);
+            launch_check(pbfint::radix_sort_pairs(grid.keys.data(), nullptr,
+                                                  grid.sortedKeys.data(),
+                                                  grid.sortedIndices.data(), grid.keys.data(),
+                                                  grid.indices.data(), n, grid.keyBits,
+                                                  grid.scratch.data(), stream));
+        }
+        {
+            Scope sc(prof, "cell_ranges", stream);
+            launch_check(pbfint::launch_fill_cells(grid.cellStart.data(), grid.cellEnd.data(),
+                                                   grid.numCells, stream));
+            launch_check(pbfint::launch_cell_ranges(grid.sortedKeys.data(), grid.cellStart.data(),
+                                                    grid.cellEnd.data(), n, stream));
+        }
+        {
+            Scope sc(prof, "gather", stream);
+            launch_check(pbfint::launch_gather(grid.sortedIndices.data(), x.data(), p.data(),
+                                               v.data(), vin.data(), id.data(), xa.data(),
+                                               pa.data(), va.data(), vina.data(), ida.data(), n,
+                                               stream));
+        }
+        std::swap(x, xa);
+        std::swap(p, pa);
+        std::swap(v, va);
+        std::swap(vin, vina);
+        std::swap(id, ida);
+    }
+
+    void find_neighbors() {
+        Scope sc(prof, "find_neighbors", stream);
+        launch_check(pbfint::launch_find_neighbors(
+            p.data(), grid.cellStart.data(), grid.cellEnd.data(), grid.dims, grid.minBounds,
+            grid.cellSize, count, params.smoothingRadius, nbr.data(), cnt.data(), kMaxNeighbors,
+            stride, status.data(), PBF_STATUS_NEIGHBOR_OVERFLOW, status.data(), stream));
+    }
+
+    // One frame: params.substeps substeps (pbf_solver.cu:186-325).
+    void enqueue_frame() {
+        const SimulationParams& P = params;
+        const std::size_t n = count;
+        const float sdt = P.dt / static_cast<float>(P.substeps);
+        const float invDt = 1.0f / sdt;
+        const float h = P.smoothingRadius;
+        const pbfdev::Colliders col = collision.state().view();
+        for (int substep = 0; substep < P.substeps; ++substep) {
+            {
+                Scope sc(prof, "predict_key", stream);
+                launch_check(pbfint::launch_predict_key(
+                    x.data(), p.data(), v.data(), vin.data(), grid.keys.data(),
+                    static_cast<int>(n), sdt, P.gravity, grid.minBounds, grid.cellSize, grid.dims,
+                    status.data(), stream));
+            }
+            for (int it = 0; it < P.solverIterations; ++it) {
+                build_and_reorder(it == 0);
+                find_neighbors();
+                {
+                    Scope sc(prof, "lambda", stream);
+                    launch_check(pbfint::launch_lambda_guarded(
+                        p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
+                        P.restDensity, P.lambdaEpsilon, lambda.data(), stride, status.data(),
+                        stream));
+                }
+                {
+                    Scope sc(prof, "delta_apply_collide", stream);
+                    launch_check(pbfint::launch_delta_apply_collide(
+                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, lambda.data(),
+                        n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
+                        stride, col, P.particleRadius, status.data(), stream));
+                }
+                std::swap(p, pa);
+            }
+            const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
+            if (post) {
+                // Rebuild so the velocity passes use the final neighbourhoods
+                // (pbf_solver.cu:260-282).
+                build_and_reorder(false);
+                find_neighbors();
+            }
+            if (P.xsphViscosity != 0.0f) {
+                Scope sc(prof, "update_xsph", stream);
+                launch_check(pbfint::launch_update_xsph(
+                    x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
+                    n, h, P.xsphViscosity, invDt, stride, status.data(), stream));
+                std::swap(v, va);
+            } else {
+                Scope sc(prof, "update_velocity", stream);
+                launch_check(pbfint::launch_update_velocity_only(x.data(), p.data(), v.data(), n,
+                                                                 invDt, status.data(), stream));
+            }
+            // positions <- predicted positions (pbf_solver.cu:210 of the kernel).
+            std::swap(x, p);
+            if (P.vorticityStrength != 0.0f) {
+                {
+                    Scope sc(prof, "vorticity", stream);
+                    launch_check(pbfint::launch_vorticity(x.data(), v.data(), nbr.data(),
+                                                          cnt.data(), kMaxNeighbors, n, h,
+                                                          omega.data(), stride, stream));
+                }
+                {
+                    Scope sc(prof, "confine_resolve", stream);
+                    launch_check(pbfint::launch_confine_resolve(
+                        x.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(), v.data(),
+                        vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
+                        P.particleRadius, P.collisionRestitution, P.collisionFriction,
+                        status.data(), stream));
+                }
+                std::swap(v, va);
+            } else {
+                Scope sc(prof, "resolve_velocities", stream);
+                launch_check(pbfint::launch_resolve_guarded(
+                    x.data(), vin.data(), v.data(), n, col, P.particleRadius,
+                    P.collisionRestitution, P.collisionFriction, status.data(), stream));
+            }
+        }
+    }
+
+    void reset_status() { launch_check(pbfint::launch_zero_u32(status.data(), 1, stream)); }
+
+    // Waits for the device and raises what the status word recorded.
+    void finish() {
+        hip_check(hipStreamSynchronize(stream));
+        prof.collect();
+        std::uint32_t st = 0;
+        hip_check(hipMemcpy(&st, status.data(), sizeof(st), hipMemcpyDeviceToHost));
+        if (st & PBF_STATUS_NEIGHBOR_OVERFLOW)
+            throw std::overflow_error("Particle neighbor count exceeds solver capacity");
+        if (st & PBF_STATUS_COLLISION_NONCONVERGENCE)
+            throw collision_nonconvergence("Collision constraints did not converge");
+    }
+
+    void copy_out(const DeviceBuffer<float4>& src, DeviceBuffer<float4>& scratch, float4* out,
+                  std::size_t n, bool toHost) {
+        launch_check(pbf_k_scatter_float4_by_stable_id(src.data(), id.data(), scratch.data(), n,
+                                                       stream));
+        if (toHost) {
+            if (out == nullptr) {
+                hip_check(hipStreamSynchronize(stream));
+                throw std::invalid_argument("Destination cannot be null");
+            }
+            hip_check(hipMemcpyAsync(out, scratch.data(), n * sizeof(float4), hipMemcpyDeviceToHost,
+                                     stream));
+        } else {
+            if (out == nullptr) {
+                hip_check(hipStreamSynchronize(stream));
+                throw std::invalid_argument("Destination cannot be null");
+            }
+            hip_check(hipMemcpyAsync(out, scratch.data(), n * sizeof(float4),
+                                     hipMemcpyDeviceToDevice, stream));
+        }
+        hip_check(hipStreamSynchronize(stream));
+    }
+};
+
+PBFSolver::PBFSolver() : state_(new State()) {}
+PBFSolver::~PBFSolver() = default;
+PBFSolver::PBFSolver(PBFSolver&&) noexcept = default;
+PBFSolver& PBFSolver::operator=(PBFSolver&&) noexcept = default;
+
+void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
+                           const SimulationParams& params) {
+    // Validation order and exception types of pbf_solver.cu:25-79.
+    if (maxParticles == 0) throw std::invalid_argument("maxParticles must be greater than zero");
+    if (maxParticles > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(kMaxNeighbors))
+        throw std::length_error("Neighbor buffer size overflows size_t");
+    if (maxParticles > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::length_error("Particle count exceeds kernel index range");
+    const std::size_t stride =
+        ((maxParticles + kBlockSize - 1) / kBlockSize) * static_cast<std::size_t>(kBlockSize);
+    if (stride > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(kMaxNeighbors))
+        throw std::length_error("Neighbor buffer size overflows size_t");
+
+    const SimulationParams& p = params;
+    if (!std::isfinite(p.dt) || p.dt <= 0.0f || !std::isfinite(p.restDensity) ||
+        p.restDensity <= 0.0f || !std::isfinite(p.particleMass) || p.particleMass <= 0.0f ||
+        !std::isfinite(p.particleRadius) || p.particleRadius <= 0.0f ||
+        !std::isfinite(p.collisionRestitution) || p.collisionRestitution < 0.0f ||
+        p.collisionRestitution > 1.0f || !std::isfinite(p.collisionFriction) ||
+        p.collisionFriction < 0.0f || p.collisionFriction > 1.0f ||
+        !std::isfinite(p.smoothingRadius) || p.smoothingRadius <= 0.0f ||
+        !std::isfinite(p.lambdaEpsilon) || p.lambdaEpsilon <= 0.0f || !std::isfinite(p.scorrK) ||
+        p.scorrK < 0.0f ||
+        (p.scorrK > 0.0f && (p.scorrN <= 0 || !std::isfinite(p.scorrDeltaQ) ||
+                             p.scorrDeltaQ < 0.0f || p.scorrDeltaQ >= p.smoothingRadius)) ||
+        !std::isfinite(p.xsphViscosity) || p.xsphViscosity < 0.0f ||
+        !std::isfinite(p.vorticityStrength) || p.vorticityStrength < 0.0f ||
+        !std::isfinite(p.gravity.x) || !std::isfinite(p.gravity.y) || !std::isfinite(p.gravity.z) ||
+        p.solverIterations <= 0 || p.substeps <= 0) {
+        throw std::invalid_argument("Invalid PBF simulation parameters");
+    }
+    const float substepDt = p.dt / static_cast<float>(p.substeps);
+    if (!std::isfinite(substepDt) || substepDt <= 0.0f || !std::isfinite(1.0f / substepDt))
+        throw std::invalid_argument("Substep duration is not representable");
+
+    const double width = static_cast<double>(maxBounds.x) - static_cast<double>(minBounds.x);
+    const double depth = static_cast<double>(maxBounds.z) - static_cast<double>(minBounds.z);
+    const double diameter = 2.0 * static_cast<double>(p.particleRadius);
+    if (width < diameter || depth < diameter)
+        throw std::invalid_argument("Particle diameter exceeds closed container dimensions");
+
+    State& s = *state_;
+    GridCore grid = GridCore::create(maxParticles, minBounds, maxBounds, p.smoothingRadius);
+
+    // From here a failed allocation leaves the solver inactive
+    // (pbf_solver.cu:84-87).
+    s.maxParticles = 0;
+    s.count = 0;
+    s.grid = std::move(grid);
+    s.collision.setContainer(Container{minBounds, maxBounds});
+    s.ensure_stream();
+
+    for (DeviceBuffer<float4>* b : {&s.x, &s.xa, &s.p, &s.pa, &s.v, &s.va, &s.vin, &s.vina, &s.omega})
+        b->allocate(maxParticles);
+    s.id.allocate(maxParticles);
+    s.ida.allocate(maxParticles);
+    s.nbr.allocate(stride * static_cast<std::size_t>(kMaxNeighbors));
+    s.cnt.allocate(maxParticles);
+    s.lambda.allocate(maxParticles);
+    s.status.allocate(1);
+
+    s.maxParticles = maxParticles;
+    s.count = 0;
+    s.stride = stride;
+    s.params = params;
+}
+
+void PBFSolver::setParticles(const float4* positions, const float4* velocities, std::size_t n) {
+    State& s = *state_;
+    if (s.maxParticles == 0)
+        throw std::logic_error("PBFSolver must be initialized before setting particles");
+    if (n > s.maxParticles) throw std::out_of_range("Particle count exceeds solver capacity");
+    if (n == 0) {
+        s.count = 0;
+        return;
+    }
+    if (positions == nullptr || velocities == nullptr)
+        throw std::invalid_argument("Particle positions and velocities must not be null");
+    for (std::size_t i = 0; i < n; ++i) {
+        if (!finite4(positions[i]) || !finite4(velocities[i]))
+            throw std::invalid_argument("Particle coordinates and velocities must be finite");
+    }
+    hip_check(hipMemcpyAsync(s.x.data(), positions, n * sizeof(float4), hipMemcpyHostToDevice,
+                             s.stream));
+    hip_check(hipMemcpyAsync(s.v.data(), velocities, n * sizeof(float4), hipMemcpyHostToDevice,
+                             s.stream));
+    launch_check(pbf_k_initialize_stable_particle_ids(s.id.data(), n, s.stream));
+    hip_check(hipStreamSynchronize(s.stream));
+    s.count = n;
+}
+
+void PBFSolver::setParticlesDevice(const float4* positions, const float4* velocities,
+                                   std::size_t n) {
+    State& s = *state_;
+    if (s.maxParticles == 0)
+        throw std::logic_error("PBFSolver must be initialized before setting particles");
+    if (n > s.maxParticles) throw std::out_of_range("Particle count exceeds solver capacity");
+    if (n == 0) {
+        s.count = 0;
+        return;
+    }
+    if (positions == nullptr || velocities == nullptr)
+        throw std::invalid_argument("Particle positions and velocities must not be null");
+    hip_check(hipDeviceSynchronize());
+    hip_check(hipMemcpyAsync(s.x.data(), positions, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                             s.stream));
+    hip_check(hipMemcpyAsync(s.v.data(), velocities, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                             s.stream));
+    launch_check(pbf_k_initialize_stable_particle_ids(s.id.data(), n, s.stream));
+    hip_check(hipStreamSynchronize(s.stream));
+    s.count = n;
+}
+
+void PBFSolver::setSpheres(const std::vector<SphereCollider>& spheres) {
+    state_->collision.setSpheres(spheres);
+}
+void PBFSolver::setBoxes(const std::vector<BoxCollider>& boxes) {
+    state_->collision.setBoxes(boxes);
+}
+void PBFSolver::setPlanes(const std::vector<PlaneCollider>& planes) {
+    state_->collision.setPlanes(planes);
+}
+
+void PBFSolver::step() {
+    State& s = *state_;
+    if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before stepping");
+    if (s.count == 0) return;
+    s.reset_status();
+    s.enqueue_frame();
+    s.finish();
+}
+
+void PBFSolver::run() {
+    if (state_->maxParticles == 0)
+        throw std::logic_error("PBFSolver must be initialized before running");
+    step();
+}
+
+void PBFSolver::run(std::size_t frameCount) {
+    State& s = *state_;
+    if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before running");
+    if (s.count == 0 || frameCount == 0) return;
+    s.reset_status();
+    for (std::size_t f = 0; f < frameCount; ++f) s.enqueue_frame();
+    s.finish();
+}
+
+void PBFSolver::copyPositionsToHost(float4* out, std::size_t n) const {
+    State& s = *state_;
+    if (n != s.count)
+        throw std::out_of_range("Requested position count does not match active particles");
+    if (n != 0) s.copy_out(s.x, s.xa, out, n, true);
+}
+
+void PBFSolver::copyVelocitiesToHost(float4* out, std::size_t n) const {
+    State& s = *state_;
+    if (n != s.count)
+        throw std::out_of_range("Requested velocity count does not match active particles");
+    if (n != 0) s.copy_out(s.v, s.va, out, n, true);
+}
+
+void PBFSolver::copyPositionsToDevice(float4* out, std::size_t n) const {
+    State& s = *state_;
+    if (n != s.count)
+        throw std::out_of_range("Requested position count does not match active particles");
+    if (n != 0) s.copy_out(s.x, s.xa, out, n, false);
+}
+
+void PBFSolver::copyVelocitiesToDevice(float4* out, std::size_t n) const {
+    State& s = *state_;
+    if (n != s.count)
+        throw std::out_of_range("Requested velocity count does not match active particles");
+    if (n != 0) s.copy_out(s.v, s.va, out, n, false);
+}
+
+std::size_t PBFSolver::particleCount() const { return state_->count; }
+std::size_t PBFSolver::maxParticles() const { return state_->maxParticles; }
+void* PBFSolver::stream() const { return state_->stream; }
+
+void PBFSolver::setProfiling(bool enabled) {
+    state_->prof.enabled = enabled;
+    state_->prof.reset();
+}
+
+std::vector<PBFSolver::KernelTime> PBFSolver::kernelTimes() const { return state_->prof.times(); }
+
+}  // namespace pbf

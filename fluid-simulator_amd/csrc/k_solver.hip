@@ -1,0 +1,823 @@
+// PBF particle kernels for gfx950: the reference's free kernels (same
+// argument meaning, slot-major neighbour lists) and the fused kernels the
+// solver step launches.
+//
+// Reference: src/pbf/integration.cu, src/pbf/pbf_solver_kernels.cu,
+// include/pbf/pbf_solver_kernels.cuh, src/pbf/collision_kernels.cu.
+#include "pbf_device.hpp"
+#include "pbf_internal.hpp"
+#include "pbf_launch.hpp"
+
+using namespace pbfdev;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Reference-shaped kernels
+// ---------------------------------------------------------------------------
+
+// integrate (src/pbf/integration.cu:6-32).
+__global__ void __launch_bounds__(kBlock) k_integrate(const pbf_float4* __restrict__ x,
+                                                      pbf_float4* __restrict__ p,
+                                                      pbf_float4* __restrict__ v, int n, float dt,
+                                                      pbf_float3 g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 vi = ld4(v, i);
+    const float4 xi = ld4(x, i);
+    vi.x = vi.x + g.x * dt;
+    vi.y = vi.y + g.y * dt;
+    vi.z = vi.z + g.z * dt;
+    float4 pi;
+    pi.x = xi.x + vi.x * dt;
+    pi.y = xi.y + vi.y * dt;
+    pi.z = xi.z + vi.z * dt;
+    pi.w = xi.w;
+    st4(v, i, vi);
+    st4(p, i, pi);
+}
+
+// computeLambda (src/pbf/pbf_solver_kernels.cu:26-105).
+__global__ void __launch_bounds__(kBlock)
+    k_lambda_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
+                  const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
+                  float eps, float* __restrict__ lambda, size_t stride,
+                  const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (status != nullptr && *status != 0u) return;
+    const float4 pi = ld4(p, i);
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        lambda[i] = qnan();
+        return;
+    }
+    const Sph s = sph_consts(h);
+    const float gs = mass / rho0;
+    // Per-neighbour terms in FP32, their sums in FP64: the density sits
+    // close to the rest density, so C = rho / rho0 - 1 cancels most of an
+    // FP32 sum's significant bits.
+    double rho = static_cast<double>(mass * s.poly_norm);
+    double gix = 0.0, giy = 0.0, giz = 0.0, sg = 0.0;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        rho += static_cast<double>(mass * poly6_d2(d2, s));
+        const float3 gw = spiky_grad(dx, dy, dz, d2, s);
+        const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
+        sg += static_cast<double>(gjx * gjx + gjy * gjy + gjz * gjz);
+        gix -= static_cast<double>(gjx);
+        giy -= static_cast<double>(gjy);
+        giz -= static_cast<double>(gjz);
+    }
+    sg += gix * gix + giy * giy + giz * giz;
+    const double C = rho / static_cast<double>(rho0) - 1.0;
+    lambda[i] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
+}
+
+// computeDeltaPosition (src/pbf/pbf_solver_kernels.cu:108-182).
+__global__ void __launch_bounds__(kBlock)
+    k_delta_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
+                 const int* __restrict__ cnt, int maxN, const float* __restrict__ lambda, size_t n,
+                 float h, float mass, float rho0, float sk, int sn, float sdq,
+                 pbf_float4* __restrict__ delta, size_t stride) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = ld4(p, i);
+    const float li = lambda[i];
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        st4(delta, i, make_float4(qnan(), qnan(), qnan(), 0.0f));
+        return;
+    }
+    const Sph s = sph_consts(h);
+    const Scorr sc = scorr_consts(h, sk, sn, sdq);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float3 gw = spiky_grad(dx, dy, dz, d2, s);
+        const float ls = li + lambda[j] + scorr_value(d2, s.h2, sc);
+        ax += ls * gw.x;
+        ay += ls * gw.y;
+        az += ls * gw.z;
+    }
+    const float cs = mass / rho0;
+    st4(delta, i, make_float4(cs * ax, cs * ay, cs * az, 0.0f));
+}
+
+// applyDeltaPosition (src/pbf/pbf_solver_kernels.cu:184-193).
+__global__ void __launch_bounds__(kBlock) k_apply_delta(pbf_float4* __restrict__ p,
+                                                        const pbf_float4* __restrict__ d,
+                                                        size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    const float4 di = ld4(d, i);
+    pi.x += di.x;
+    pi.y += di.y;
+    pi.z += di.z;
+    st4(p, i, pi);
+}
+
+// updateVelocityAndPosition (src/pbf/pbf_solver_kernels.cu:195-211).
+__global__ void __launch_bounds__(kBlock) k_update_velocity(pbf_float4* __restrict__ x,
+                                                            const pbf_float4* __restrict__ p,
+                                                            pbf_float4* __restrict__ v, size_t n,
+                                                            float inv_dt) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = ld4(x, i);
+    const float4 pi = ld4(p, i);
+    float4 vi = ld4(v, i);
+    vi.x = (pi.x - xi.x) * inv_dt;
+    vi.y = (pi.y - xi.y) * inv_dt;
+    vi.z = (pi.z - xi.z) * inv_dt;
+    st4(v, i, vi);
+    st4(x, i, pi);
+}
+
+// applyXsphViscosity (src/pbf/pbf_solver_kernels.cu:213-266).
+__global__ void __launch_bounds__(kBlock)
+    k_xsph_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
+                const int* __restrict__ cnt, int maxN, const pbf_float4* __restrict__ vin,
+                pbf_float4* __restrict__ vout, size_t n, float h, float visc, size_t stride) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 vi = ld4(vin, i);
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        st4(vout, i, make_float4(qnan(), qnan(), qnan(), vi.w));
+        return;
+    }
+    if (visc == 0.0f) {
+        st4(vout, i, vi);
+        return;
+    }
+    const Sph s = sph_consts(h);
+    const float4 pi = ld4(p, i);
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float4 vj = ld4(vin, j);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
+        cx += (vj.x - vi.x) * w;
+        cy += (vj.y - vi.y) * w;
+        cz += (vj.z - vi.z) * w;
+    }
+    st4(vout, i, make_float4(vi.x + visc * cx, vi.y + visc * cy, vi.z + visc * cz, vi.w));
+}
+
+// computeVorticity (src/pbf/pbf_solver_kernels.cu:268-315).
+__global__ void __launch_bounds__(kBlock)
+    k_vorticity_list(const pbf_float4* __restrict__ p, const pbf_float4* __restrict__ v,
+                     const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
+                     size_t n, float h, pbf_float4* __restrict__ omega, size_t stride) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        st4(omega, i, make_float4(qnan(), qnan(), qnan(), 0.0f));
+        return;
+    }
+    const Sph s = sph_consts(h);
+    const float4 pi = ld4(p, i);
+    const float4 vi = ld4(v, i);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float4 vj = ld4(v, j);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float3 g = spiky_grad(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+        ox += g.y * uz - g.z * uy;
+        oy += g.z * ux - g.x * uz;
+        oz += g.x * uy - g.y * ux;
+    }
+    st4(omega, i, make_float4(ox, oy, oz, 0.0f));
+}
+
+// applyVorticityConfinement (src/pbf/pbf_solver_kernels.cu:317-385) as a
+// device function, shared by the API kernel and the fused confinement +
+// contact kernel.  Returns false for an invalid neighbour count.
+__device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__ p,
+                                        const uint32_t* __restrict__ nbr, int c,
+                                        const pbf_float4* __restrict__ omega, float4 vi,
+                                        const Sph& s, float dt, float strength, size_t stride,
+                                        float4& out) {
+    const float4 pi = ld4(p, i);
+    const float4 wi = ld4(omega, i);
+    const float wil = __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float4 wj = ld4(omega, j);
+        const float wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float3 g = spiky_grad(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+        const float md = wjl - wil;
+        ex += md * g.x;
+        ey += md * g.y;
+        ez += md * g.z;
+    }
+    const float el = __builtin_sqrtf(ex * ex + ey * ey + ez * ez);
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (el > 1.0e-6f) {  // normalizationEpsilon, pbf_solver_kernels.cu:367
+        nx = ex / el;
+        ny = ey / el;
+        nz = ez / el;
+    }
+    const float fx = strength * (ny * wi.z - nz * wi.y);
+    const float fy = strength * (nz * wi.x - nx * wi.z);
+    const float fz = strength * (nx * wi.y - ny * wi.x);
+    out = make_float4(vi.x + dt * fx, vi.y + dt * fy, vi.z + dt * fz, vi.w);
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_confine_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
+                   const int* __restrict__ cnt, int maxN, const pbf_float4* __restrict__ omega,
+                   const pbf_float4* __restrict__ vin, pbf_float4* __restrict__ vout, size_t n,
+                   float h, float dt, float strength, size_t stride) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 vi = ld4(vin, i);
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        st4(vout, i, make_float4(qnan(), qnan(), qnan(), vi.w));
+        return;
+    }
+    const Sph s = sph_consts(h);
+    float4 out;
+    confine(i, p, nbr, c, omega, vi, s, dt, strength, stride, out);
+    st4(vout, i, out);
+}
+
+// gatherSolverState (src/pbf/pbf_solver_kernels.cu:387-407).
+__global__ void __launch_bounds__(kBlock)
+    k_gather(const uint32_t* __restrict__ idx, const pbf_float4* __restrict__ x,
+             const pbf_float4* __restrict__ p, const pbf_float4* __restrict__ v,
+             const pbf_float4* __restrict__ vin, const uint32_t* __restrict__ id,
+             pbf_float4* __restrict__ xo, pbf_float4* __restrict__ po, pbf_float4* __restrict__ vo,
+             pbf_float4* __restrict__ vino, uint32_t* __restrict__ ido, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = idx[i];
+    st4(xo, i, ld4(x, s));
+    st4(po, i, ld4(p, s));
+    st4(vo, i, ld4(v, s));
+    st4(vino, i, ld4(vin, s));
+    ido[i] = id[s];
+}
+
+// initializeStableParticleIds (src/pbf/pbf_solver_kernels.cu:409-416).
+__global__ void __launch_bounds__(kBlock) k_init_ids(uint32_t* __restrict__ id, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) id[i] = static_cast<uint32_t>(i);
+}
+
+// scatterFloat4ByStableId (src/pbf/pbf_solver_kernels.cu:418-427).
+__global__ void __launch_bounds__(kBlock) k_scatter_by_id(const pbf_float4* __restrict__ vals,
+                                                          const uint32_t* __restrict__ id,
+                                                          pbf_float4* __restrict__ out, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) st4(out, id[i], ld4(vals, i));
+}
+
+// Collision kernels (src/pbf/collision_kernels.cu:47-234): one projection
+// per launch, as the reference's CollisionSystem passes launch them.
+__global__ void __launch_bounds__(kBlock) k_solve_container(pbf_float4* __restrict__ p, size_t n,
+                                                            const pbf_container* __restrict__ c,
+                                                            float r, int* __restrict__ flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    const pbf_container k = *c;
+    if (project_container(pi, k, r) && flag != nullptr) atomicOr(flag, 1);
+    st4(p, i, pi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_solve_spheres(pbf_float4* __restrict__ p, size_t n,
+                                                          const pbf_sphere* __restrict__ s,
+                                                          size_t count, float r,
+                                                          int* __restrict__ flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    bool marked = false;
+    for (size_t k = 0; k < count; ++k) marked |= project_sphere(pi, s[k], r);
+    if (marked && flag != nullptr) atomicOr(flag, 1);
+    st4(p, i, pi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_solve_boxes(pbf_float4* __restrict__ p, size_t n,
+                                                        const pbf_box* __restrict__ b,
+                                                        size_t count, float r,
+                                                        int* __restrict__ flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    bool marked = false;
+    for (size_t k = 0; k < count; ++k) marked |= project_box(pi, b[k], r);
+    if (marked && flag != nullptr) atomicOr(flag, 1);
+    st4(p, i, pi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_solve_planes(pbf_float4* __restrict__ p, size_t n,
+                                                         const pbf_plane* __restrict__ pl,
+                                                         size_t count, float r,
+                                                         int* __restrict__ flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    bool marked = false;
+    for (size_t k = 0; k < count; ++k) marked |= project_plane(pi, pl[k], r);
+    if (marked && flag != nullptr) atomicOr(flag, 1);
+    st4(p, i, pi);
+}
+
+// resolveVelocitiesKernel (src/pbf/collision_kernels.cu:236-371).
+__global__ void __launch_bounds__(kBlock)
+    k_resolve_velocities(const pbf_float4* __restrict__ x, const pbf_float4* vin,
+                         pbf_float4* v, size_t n, Colliders col, float r, float e, float f) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = ld4(x, i);
+    const float4 vini = ld4(vin, i);
+    float4 vi = ld4(v, i);
+    resolve_velocity(xi, vini, vi, col, r, e, f);
+    st4(v, i, vi);
+}
+
+// Batched device evaluation of the inline helpers.
+__global__ void k_poly6(const pbf_float3* __restrict__ d, size_t n, float h, float* __restrict__ out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Sph s = sph_consts(h);
+    const pbf_float3 q = d[i];
+    out[i] = poly6_d2(q.x * q.x + q.y * q.y + q.z * q.z, s);
+}
+
+__global__ void k_spiky(const pbf_float3* __restrict__ d, size_t n, float h,
+                        pbf_float3* __restrict__ out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Sph s = sph_consts(h);
+    const pbf_float3 q = d[i];
+    const float3 g = spiky_grad(q.x, q.y, q.z, q.x * q.x + q.y * q.y + q.z * q.z, s);
+    out[i] = pbf_float3{g.x, g.y, g.z};
+}
+
+__global__ void k_scorr(const float* __restrict__ d2, size_t n, float h, float k, int sn, float dq,
+                        float* __restrict__ out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Scorr c = scorr_consts(h, k, sn, dq);
+    out[i] = scorr_value(d2[i], h * h, c);
+}
+
+// ---------------------------------------------------------------------------
+// Fused solver kernels
+// ---------------------------------------------------------------------------
+
+// integrate + collisionInputVelocities snapshot + cell key of p*
+// (pbf_solver.cu:199-208 and spatial_grid.cu:100-120 in one pass).
+__global__ void __launch_bounds__(kBlock)
+    k_predict_key(const pbf_float4* __restrict__ x, pbf_float4* __restrict__ p,
+                  pbf_float4* __restrict__ v, pbf_float4* __restrict__ vin,
+                  uint32_t* __restrict__ key, int n, float dt, pbf_float3 g, pbf_float3 mn,
+                  float cs, pbf_int3 gs, const uint32_t* __restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    float4 vi = ld4(v, i);
+    const float4 xi = ld4(x, i);
+    vi.x = vi.x + g.x * dt;
+    vi.y = vi.y + g.y * dt;
+    vi.z = vi.z + g.z * dt;
+    float4 pi;
+    pi.x = xi.x + vi.x * dt;
+    pi.y = xi.y + vi.y * dt;
+    pi.z = xi.z + vi.z * dt;
+    pi.w = xi.w;
+    st4(v, i, vi);
+    st4(vin, i, vi);
+    st4(p, i, pi);
+    key[i] = hash_key(pi, mn, cs, gs);
+}
+
+// Cell keys of the current working set (computeParticleKeys,
+// spatial_grid.cu:100-120); the slot index is implicit.
+__global__ void __launch_bounds__(kBlock)
+    k_keys(const pbf_float4* __restrict__ p, uint32_t* __restrict__ key, size_t n, pbf_float3 mn,
+           float cs, pbf_int3 gs) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = hash_key(ld4(p, i), mn, cs, gs);
+}
+
+// computeDeltaPosition + applyDeltaPosition + CollisionSystem::solve for one
+// constraint iteration (pbf_solver.cu:240-257).  Reads p* from pin and writes
+// the corrected, projected p* to pout (Jacobi), so every thread sees the
+// positions the reference's separate Delta-p kernel sees.
+__global__ void __launch_bounds__(kBlock)
+    k_delta_apply_collide(const pbf_float4* __restrict__ pin, pbf_float4* __restrict__ pout,
+                          const uint32_t* __restrict__ nbr, const int* __restrict__ cnt,
+                          int maxN, const float* __restrict__ lambda, size_t n, float h,
+                          float mass, float rho0, float sk, int sn, float sdq, size_t stride,
+                          Colliders col, float radius, uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    float4 pi = ld4(pin, i);
+    const int c = cnt[i];
+    float dpx, dpy, dpz;
+    if (!valid_count(c, maxN)) {
+        dpx = dpy = dpz = qnan();
+    } else {
+        const Sph s = sph_consts(h);
+        const Scorr sc = scorr_consts(h, sk, sn, sdq);
+        const float li = lambda[i];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (int q = 0; q < c; ++q) {
+            const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+            const float4 pj = ld4(pin, j);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float3 gw = spiky_grad(dx, dy, dz, d2, s);
+            const float ls = li + lambda[j] + scorr_value(d2, s.h2, sc);
+            ax += ls * gw.x;
+            ay += ls * gw.y;
+            az += ls * gw.z;
+        }
+        const float cs = mass / rho0;
+        dpx = cs * ax;
+        dpy = cs * ay;
+        dpz = cs * az;
+    }
+    pi.x += dpx;
+    pi.y += dpy;
+    pi.z += dpz;
+    if (!collision_fixed_point(pi, col, radius))
+        atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
+    st4(pout, i, pi);
+}
+
+// updateVelocityAndPosition + applyXsphViscosity (pbf_solver.cu:284-298).
+// Velocities v = (p* - x) / dt are rebuilt on the fly for i and every
+// neighbour with the same FP32 expression the reference stores, so the
+// stored intermediate velocity buffer is not needed.  x <- p* is a buffer
+// swap on the host.
+__global__ void __launch_bounds__(kBlock)
+    k_update_xsph(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ p,
+                  const pbf_float4* __restrict__ vold, pbf_float4* __restrict__ vout,
+                  const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
+                  size_t n, float h, float visc, float inv_dt, size_t stride,
+                  const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    const float4 xi = ld4(x, i);
+    const float4 pi = ld4(p, i);
+    const float vw = vold[i].w;
+    const float vix = (pi.x - xi.x) * inv_dt;
+    const float viy = (pi.y - xi.y) * inv_dt;
+    const float viz = (pi.z - xi.z) * inv_dt;
+    const int c = cnt[i];
+    if (!valid_count(c, maxN)) {
+        st4(vout, i, make_float4(qnan(), qnan(), qnan(), vw));
+        return;
+    }
+    const Sph s = sph_consts(h);
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int q = 0; q < c; ++q) {
+        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
+        const float4 pj = ld4(p, j);
+        const float4 xj = ld4(x, j);
+        const float vjx = (pj.x - xj.x) * inv_dt;
+        const float vjy = (pj.y - xj.y) * inv_dt;
+        const float vjz = (pj.z - xj.z) * inv_dt;
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
+        cx += (vjx - vix) * w;
+        cy += (vjy - viy) * w;
+        cz += (vjz - viz) * w;
+    }
+    st4(vout, i, make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw));
+}
+
+// updateVelocityAndPosition without XSPH, velocity only (x <- p* is a swap).
+__global__ void __launch_bounds__(kBlock)
+    k_update_velocity_only(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ p,
+                           pbf_float4* __restrict__ v, size_t n, float inv_dt,
+                           const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    const float4 xi = ld4(x, i);
+    const float4 pi = ld4(p, i);
+    float4 vi = ld4(v, i);
+    vi.x = (pi.x - xi.x) * inv_dt;
+    vi.y = (pi.y - xi.y) * inv_dt;
+    vi.z = (pi.z - xi.z) * inv_dt;
+    st4(v, i, vi);
+}
+
+// applyVorticityConfinement + resolveVelocitiesKernel (pbf_solver.cu:309-323).
+__global__ void __launch_bounds__(kBlock)
+    k_confine_resolve(const pbf_float4* __restrict__ x, const uint32_t* __restrict__ nbr,
+                      const int* __restrict__ cnt, int maxN, const pbf_float4* __restrict__ omega,
+                      const pbf_float4* __restrict__ vcur, const pbf_float4* __restrict__ vsnap,
+                      pbf_float4* __restrict__ vout, size_t n, float h, float dt, float strength,
+                      size_t stride, Colliders col, float radius, float e, float f,
+                      const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    const float4 vi = ld4(vcur, i);
+    const int c = cnt[i];
+    float4 out;
+    if (!valid_count(c, maxN)) {
+        out = make_float4(qnan(), qnan(), qnan(), vi.w);
+    } else {
+        const Sph s = sph_consts(h);
+        confine(i, x, nbr, c, omega, vi, s, dt, strength, stride, out);
+    }
+    resolve_velocity(ld4(x, i), ld4(vsnap, i), out, col, radius, e, f);
+    st4(vout, i, out);
+}
+
+// Clears words with a kernel rather than a memset node, so the step stays
+// capturable in a graph.
+__global__ void __launch_bounds__(kBlock) k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// CollisionSystem::solve (collision_system.cu:133-172) in one launch.
+__global__ void __launch_bounds__(kBlock)
+    k_collide_fixed_point(pbf_float4* __restrict__ p, size_t n, Colliders col, float radius,
+                          uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pi = ld4(p, i);
+    if (!collision_fixed_point(pi, col, radius))
+        atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
+    st4(p, i, pi);
+}
+
+// resolveVelocitiesKernel as the last pass when confinement is off.
+__global__ void __launch_bounds__(kBlock)
+    k_resolve_guarded(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ vsnap,
+                      pbf_float4* __restrict__ v, size_t n, Colliders col, float r, float e,
+                      float f, const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    float4 vi = ld4(v, i);
+    resolve_velocity(ld4(x, i), ld4(vsnap, i), vi, col, r, e, f);
+    st4(v, i, vi);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI launchers (reference kernels)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+pbf_status pbf_k_integrate(const pbf_float4* x, pbf_float4* p, pbf_float4* v, int n, float dt,
+                           pbf_float3 g, void* stream) {
+    if (n <= 0) return PBF_OK;
+    PBF_LAUNCH(k_integrate, grid_for(n), kBlock, stream, x, p, v, n, dt, g);
+}
+
+pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
+                                int maxN, size_t n, float h, float mass, float rho0, float eps,
+                                float* lambda, size_t stride, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_lambda_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass, rho0,
+               eps, lambda, stride, static_cast<const uint32_t*>(nullptr));
+}
+
+pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
+                                        int maxN, const float* lambda, size_t n, float h,
+                                        float mass, float rho0, float sk, int sn, float sdq,
+                                        pbf_float4* delta, size_t stride, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_delta_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, lambda, n, h, mass,
+               rho0, sk, sn, sdq, delta, stride);
+}
+
+pbf_status pbf_k_apply_delta_position(pbf_float4* p, const pbf_float4* d, size_t n, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_apply_delta, grid_for(n), kBlock, stream, p, d, n);
+}
+
+pbf_status pbf_k_update_velocity_and_position(pbf_float4* x, const pbf_float4* p, pbf_float4* v,
+                                              size_t n, float inv_dt, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_update_velocity, grid_for(n), kBlock, stream, x, p, v, n, inv_dt);
+}
+
+pbf_status pbf_k_apply_xsph_viscosity(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
+                                      int maxN, const pbf_float4* vin, pbf_float4* vout, size_t n,
+                                      float h, float visc, size_t stride, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_xsph_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, vin, vout, n, h, visc,
+               stride);
+}
+
+pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
+                                   const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
+                                   size_t stride, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_vorticity_list, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h, omega,
+               stride);
+}
+
+pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
+                                             const int* cnt, int maxN, const pbf_float4* omega,
+                                             const pbf_float4* vin, pbf_float4* vout, size_t n,
+                                             float h, float dt, float strength, size_t stride,
+                                             void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_confine_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, omega, vin, vout,
+               n, h, dt, strength, stride);
+}
+
+pbf_status pbf_k_gather_solver_state(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
+                                     const pbf_float4* v, const pbf_float4* vin,
+                                     const uint32_t* id, pbf_float4* xo, pbf_float4* po,
+                                     pbf_float4* vo, pbf_float4* vino, uint32_t* ido, size_t n,
+                                     void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_gather, grid_for(n), kBlock, stream, idx, x, p, v, vin, id, xo, po, vo, vino, ido,
+               n);
+}
+
+pbf_status pbf_k_initialize_stable_particle_ids(uint32_t* id, size_t n, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_init_ids, grid_for(n), kBlock, stream, id, n);
+}
+
+pbf_status pbf_k_scatter_float4_by_stable_id(const pbf_float4* vals, const uint32_t* id,
+                                             pbf_float4* out, size_t n, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_scatter_by_id, grid_for(n), kBlock, stream, vals, id, out, n);
+}
+
+pbf_status pbf_k_solve_container(pbf_float4* p, size_t n, const pbf_container* c, float r,
+                                 int* flag, void* stream) {
+    if (n == 0) return PBF_OK;
+    if (p == nullptr || c == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null pointer");
+    PBF_LAUNCH(k_solve_container, grid_for(n), kBlock, stream, p, n, c, r, flag);
+}
+
+pbf_status pbf_k_solve_spheres(pbf_float4* p, size_t n, const pbf_sphere* s, size_t count, float r,
+                               int* flag, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_solve_spheres, grid_for(n), kBlock, stream, p, n, s, count, r, flag);
+}
+
+pbf_status pbf_k_solve_boxes(pbf_float4* p, size_t n, const pbf_box* b, size_t count, float r,
+                             int* flag, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_solve_boxes, grid_for(n), kBlock, stream, p, n, b, count, r, flag);
+}
+
+pbf_status pbf_k_solve_planes(pbf_float4* p, size_t n, const pbf_plane* pl, size_t count, float r,
+                              int* flag, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_solve_planes, grid_for(n), kBlock, stream, p, n, pl, count, r, flag);
+}
+
+pbf_status pbf_k_resolve_velocities(const pbf_float4* x, const pbf_float4* vin, pbf_float4* v,
+                                    size_t n, const pbf_container* c, const pbf_sphere* s,
+                                    size_t ns, const pbf_box* b, size_t nb, const pbf_plane* pl,
+                                    size_t np, float r, float e, float f, void* stream) {
+    if (n == 0) return PBF_OK;
+    if (c == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "container must not be null");
+    const Colliders col{c, s, b, pl, static_cast<int>(ns), static_cast<int>(nb),
+                        static_cast<int>(np)};
+    PBF_LAUNCH(k_resolve_velocities, grid_for(n), kBlock, stream, x, vin, v, n, col, r, e, f);
+}
+
+pbf_status pbf_k_poly6(const pbf_float3* d, size_t n, float h, float* out, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_poly6, grid_for(n), kBlock, stream, d, n, h, out);
+}
+
+pbf_status pbf_k_spiky_gradient(const pbf_float3* d, size_t n, float h, pbf_float3* out,
+                                void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_spiky, grid_for(n), kBlock, stream, d, n, h, out);
+}
+
+pbf_status pbf_k_artificial_pressure(const float* d2, size_t n, float h, float k, int sn, float dq,
+                                     float* out, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_scorr, grid_for(n), kBlock, stream, d2, n, h, k, sn, dq, out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Internal launchers of the fused step kernels (pbf_internal.hpp)
+// ---------------------------------------------------------------------------
+namespace pbfint {
+
+pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
+                              uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
+                              float cs, pbf_int3 gs, const uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_predict_key, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g, mn, cs,
+               gs, status);
+}
+
+pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const Colliders& col, float radius,
+                                      uint32_t* status, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_collide_fixed_point, grid_for(n), kBlock, stream, p, n, col, radius, status);
+}
+
+pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream) {
+    PBF_LAUNCH(k_zero_u32, grid_for(n), kBlock, stream, p, n);
+}
+
+pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
+                       pbf_int3 gs, void* stream) {
+    PBF_LAUNCH(k_keys, grid_for(n), kBlock, stream, p, key, n, mn, cs, gs);
+}
+
+pbf_status launch_lambda_guarded(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
+                                 int maxN, size_t n, float h, float mass, float rho0, float eps,
+                                 float* lambda, size_t stride, const uint32_t* status,
+                                 void* stream) {
+    PBF_LAUNCH(k_lambda_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass, rho0,
+               eps, lambda, stride, status);
+}
+
+pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
+                                      const int* cnt, int maxN, const float* lambda, size_t n,
+                                      float h, float mass, float rho0, float sk, int sn, float sdq,
+                                      size_t stride, const Colliders& col, float radius,
+                                      uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_delta_apply_collide, grid_for(n), kBlock, stream, pin, pout, nbr, cnt, maxN,
+               lambda, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, status);
+}
+
+pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
+                              pbf_float4* vout, const uint32_t* nbr, const int* cnt, int maxN,
+                              size_t n, float h, float visc, float inv_dt, size_t stride,
+                              const uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_update_xsph, grid_for(n), kBlock, stream, x, p, vold, vout, nbr, cnt, maxN, n, h,
+               visc, inv_dt, stride, status);
+}
+
+pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
+                                       size_t n, float inv_dt, const uint32_t* status,
+                                       void* stream) {
+    PBF_LAUNCH(k_update_velocity_only, grid_for(n), kBlock, stream, x, p, v, n, inv_dt, status);
+}
+
+pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
+                            const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
+                            size_t stride, void* stream) {
+    PBF_LAUNCH(k_vorticity_list, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h, omega,
+               stride);
+}
+
+pbf_status launch_confine_resolve(const pbf_float4* x, const uint32_t* nbr, const int* cnt,
+                                  int maxN, const pbf_float4* omega, const pbf_float4* vcur,
+                                  const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
+                                  float dt, float strength, size_t stride, const Colliders& col,
+                                  float radius, float e, float f, const uint32_t* status,
+                                  void* stream) {
+    PBF_LAUNCH(k_confine_resolve, grid_for(n), kBlock, stream, x, nbr, cnt, maxN, omega, vcur,
+               vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, status);
+}
+
+pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,
+                                  size_t n, const Colliders& col, float r, float e, float f,
+                                  const uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_resolve_guarded, grid_for(n), kBlock, stream, x, vsnap, v, n, col, r, e, f,
+               status);
+}
+
+pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
+                         const pbf_float4* v, const pbf_float4* vin, const uint32_t* id,
+                         pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
+                         uint32_t* ido, size_t n, void* stream) {
+    PBF_LAUNCH(k_gather, grid_for(n), kBlock, stream, idx, x, p, v, vin, id, xo, po, vo, vino, ido,
+               n);
+}
+
+}  // namespace pbfint

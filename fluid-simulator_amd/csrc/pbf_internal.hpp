@@ -1,0 +1,72 @@
+// Library-internal launchers of the fused step kernels and the grid build.
+// The public kernel launchers are declared in pbf_amd.h.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "pbf_amd.h"
+#include "pbf_colliders.hpp"
+
+namespace pbfint {
+
+// Grid (k_grid.hip)
+long long stencil_radius(float h, float cellSize);
+pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* ce, pbf_int3 g,
+                                 pbf_float3 mn, float cellSize, size_t n, float h,
+                                 uint32_t* nbr, int* cnt, int maxN, size_t stride,
+                                 uint32_t* flag, uint32_t bit, const uint32_t* guard,
+                                 void* stream);
+pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
+pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream);
+size_t radix_scratch_words(size_t n);
+int radix_key_bits(size_t numCells);
+pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                            uint32_t* vout, uint32_t* tmpK, uint32_t* tmpV, size_t n, int bits,
+                            uint32_t* scratch, void* stream);
+
+// Solver (k_solver.hip)
+pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
+                              uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
+                              float cs, pbf_int3 gs, const uint32_t* status, void* stream);
+pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
+                       pbf_int3 gs, void* stream);
+pbf_status launch_lambda_guarded(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
+                                 int maxN, size_t n, float h, float mass, float rho0, float eps,
+                                 float* lambda, size_t stride, const uint32_t* status,
+                                 void* stream);
+pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
+                                      const int* cnt, int maxN, const float* lambda, size_t n,
+                                      float h, float mass, float rho0, float sk, int sn, float sdq,
+                                      size_t stride, const pbfdev::Colliders& col, float radius,
+                                      uint32_t* status, void* stream);
+pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
+                              pbf_float4* vout, const uint32_t* nbr, const int* cnt, int maxN,
+                              size_t n, float h, float visc, float inv_dt, size_t stride,
+                              const uint32_t* status, void* stream);
+pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
+                                       size_t n, float inv_dt, const uint32_t* status,
+                                       void* stream);
+pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
+                            const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
+                            size_t stride, void* stream);
+pbf_status launch_confine_resolve(const pbf_float4* x, const uint32_t* nbr, const int* cnt,
+                                  int maxN, const pbf_float4* omega, const pbf_float4* vcur,
+                                  const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
+                                  float dt, float strength, size_t stride,
+                                  const pbfdev::Colliders& col, float radius, float e, float f,
+                                  const uint32_t* status, void* stream);
+pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,
+                                  size_t n, const pbfdev::Colliders& col, float r, float e,
+                                  float f, const uint32_t* status, void* stream);
+pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
+                         const pbf_float4* v, const pbf_float4* vin, const uint32_t* id,
+                         pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
+                         uint32_t* ido, size_t n, void* stream);
+pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream);
+// CollisionSystem::solve as one launch: per-particle fixed point, at most 32
+// passes; sets PBF_STATUS_COLLISION_NONCONVERGENCE in *status otherwise.
+pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const pbfdev::Colliders& col,
+                                      float radius, uint32_t* status, void* stream);
+
+}  // namespace pbfint

@@ -1,0 +1,602 @@
+"""Python binding of libpbf_amd.so, the gfx950 Position Based Fluids library.
+
+The classes mirror the reference C++ API of DusanCojic/Fluid-Simulator
+(include/pbf/pbf_solver.hpp, spatial_grid.cuh, collision_system.hpp): same
+member names, argument meaning and error behaviour.  C++ exception types map
+onto Python exceptions:
+
+    std::invalid_argument -> InvalidArgument (ValueError)
+    std::logic_error      -> LogicError (RuntimeError)
+    std::out_of_range     -> OutOfRange (IndexError)
+    std::length_error     -> LengthError (ValueError)
+    std::overflow_error   -> NeighborOverflow (OverflowError)
+    std::runtime_error    -> CollisionNonConvergence / DeviceError (RuntimeError)
+
+Float4 arrays are numpy float32 arrays of shape (n, 4).  Everything runs
+through the HIP library; there is no CPU path, and importing fails loudly
+when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "lib", "LIB_PATH", "float3", "int3", "SimulationParams", "Container", "SphereCollider",
+    "BoxCollider", "PlaneCollider", "make_float3", "DeviceArray", "PBFSolver", "SpatialGrid",
+    "CollisionSystem", "kernels", "PbfError", "InvalidArgument", "LogicError", "OutOfRange",
+    "LengthError", "NeighborOverflow", "CollisionNonConvergence", "DeviceError",
+    "STATUS_NEIGHBOR_OVERFLOW", "STATUS_COLLISION_NONCONVERGENCE",
+]
+
+LIB_PATH = Path(__file__).resolve().parent / "libpbf_amd.so"
+
+STATUS_NEIGHBOR_OVERFLOW = 0x1
+STATUS_COLLISION_NONCONVERGENCE = 0x2
+
+
+class PbfError(Exception):
+    """Base class of the library's errors."""
+
+
+class InvalidArgument(PbfError, ValueError):
+    pass
+
+
+class LogicError(PbfError, RuntimeError):
+    pass
+
+
+class OutOfRange(PbfError, IndexError):
+    pass
+
+
+class LengthError(PbfError, ValueError):
+    pass
+
+
+class NeighborOverflow(PbfError, OverflowError):
+    pass
+
+
+class CollisionNonConvergence(PbfError, RuntimeError):
+    pass
+
+
+class DeviceError(PbfError, RuntimeError):
+    pass
+
+
+_ERRORS = {1: InvalidArgument, 2: LogicError, 3: OutOfRange, 4: LengthError,
+           5: NeighborOverflow, 6: CollisionNonConvergence, 7: DeviceError}
+
+
+# --------------------------------------------------------------------------
+# Plain data types (layouts of pbf_amd.h)
+# --------------------------------------------------------------------------
+class float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+    def __repr__(self):
+        return f"float3({self.x}, {self.y}, {self.z})"
+
+
+class int3(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+
+def make_float3(x, y, z) -> float3:
+    return float3(float(x), float(y), float(z))
+
+
+def _f3(v) -> float3:
+    if isinstance(v, float3):
+        return v
+    x, y, z = v
+    return float3(float(x), float(y), float(z))
+
+
+class SimulationParams(C.Structure):
+    """SimulationParams (include/pbf/simulation_params.hpp:6-29), 72 bytes."""
+    _fields_ = [("dt", C.c_float), ("restDensity", C.c_float), ("particleMass", C.c_float),
+                ("particleRadius", C.c_float), ("collisionRestitution", C.c_float),
+                ("collisionFriction", C.c_float), ("smoothingRadius", C.c_float),
+                ("lambdaEpsilon", C.c_float), ("solverIterations", C.c_int),
+                ("substeps", C.c_int), ("gravity", float3), ("scorrK", C.c_float),
+                ("scorrN", C.c_int), ("scorrDeltaQ", C.c_float), ("xsphViscosity", C.c_float),
+                ("vorticityStrength", C.c_float)]
+
+
+class Container(C.Structure):
+    _fields_ = [("min", float3), ("max", float3)]
+
+
+class SphereCollider(C.Structure):
+    _fields_ = [("center", float3), ("radius", C.c_float)]
+
+
+class BoxCollider(C.Structure):
+    _fields_ = [("center", float3), ("halfExtents", float3)]
+
+
+class PlaneCollider(C.Structure):
+    _fields_ = [("point", float3), ("normal", float3)]
+
+
+assert C.sizeof(SimulationParams) == 72
+assert C.sizeof(Container) == 24 and C.sizeof(SphereCollider) == 16
+assert C.sizeof(BoxCollider) == 24 and C.sizeof(PlaneCollider) == 24
+
+
+def _sphere(s):
+    if isinstance(s, SphereCollider):
+        return s
+    c, r = s
+    return SphereCollider(_f3(c), float(r))
+
+
+def _box(b):
+    if isinstance(b, BoxCollider):
+        return b
+    c, h = b
+    return BoxCollider(_f3(c), _f3(h))
+
+
+def _plane(p):
+    if isinstance(p, PlaneCollider):
+        return p
+    pt, nrm = p
+    return PlaneCollider(_f3(pt), _f3(nrm))
+
+
+# --------------------------------------------------------------------------
+# Library loading and signatures
+# --------------------------------------------------------------------------
+_vp = C.c_void_p
+_sz = C.c_size_t
+_f = C.c_float
+_i = C.c_int
+_st = C.c_int
+
+_SIGNATURES = {
+    "pbf_last_error": (C.c_char_p, []),
+    "pbf_version": (C.c_char_p, []),
+    "pbf_device_alloc": (_st, [C.POINTER(_vp), _sz]),
+    "pbf_device_free": (_st, [_vp]),
+    "pbf_memcpy_htod": (_st, [_vp, _vp, _sz]),
+    "pbf_memcpy_dtoh": (_st, [_vp, _vp, _sz]),
+    "pbf_memcpy_dtod": (_st, [_vp, _vp, _sz]),
+    "pbf_memset": (_st, [_vp, _i, _sz]),
+    "pbf_device_synchronize": (_st, []),
+    "pbf_stream_create": (_st, [C.POINTER(_vp)]),
+    "pbf_stream_destroy": (_st, [_vp]),
+    "pbf_stream_synchronize": (_st, [_vp]),
+    "pbf_k_integrate": (_st, [_vp, _vp, _vp, _i, _f, float3, _vp]),
+    "pbf_k_find_neighbors": (_st, [_vp, _vp, _vp, int3, float3, _f, _sz, _f, _vp, _vp, _i, _sz,
+                                   _vp, _vp]),
+    "pbf_k_compute_lambda": (_st, [_vp, _vp, _vp, _i, _sz, _f, _f, _f, _f, _vp, _sz, _vp]),
+    "pbf_k_compute_delta_position": (_st, [_vp, _vp, _vp, _i, _vp, _sz, _f, _f, _f, _f, _i, _f,
+                                           _vp, _sz, _vp]),
+    "pbf_k_apply_delta_position": (_st, [_vp, _vp, _sz, _vp]),
+    "pbf_k_update_velocity_and_position": (_st, [_vp, _vp, _vp, _sz, _f, _vp]),
+    "pbf_k_apply_xsph_viscosity": (_st, [_vp, _vp, _vp, _i, _vp, _vp, _sz, _f, _f, _sz, _vp]),
+    "pbf_k_compute_vorticity": (_st, [_vp, _vp, _vp, _vp, _i, _sz, _f, _vp, _sz, _vp]),
+    "pbf_k_apply_vorticity_confinement": (_st, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _sz, _f, _f, _f,
+                                                _sz, _vp]),
+    "pbf_k_gather_solver_state": (_st, [_vp] * 11 + [_sz, _vp]),
+    "pbf_k_initialize_stable_particle_ids": (_st, [_vp, _sz, _vp]),
+    "pbf_k_scatter_float4_by_stable_id": (_st, [_vp, _vp, _vp, _sz, _vp]),
+    "pbf_k_solve_container": (_st, [_vp, _sz, _vp, _f, _vp, _vp]),
+    "pbf_k_solve_spheres": (_st, [_vp, _sz, _vp, _sz, _f, _vp, _vp]),
+    "pbf_k_solve_boxes": (_st, [_vp, _sz, _vp, _sz, _f, _vp, _vp]),
+    "pbf_k_solve_planes": (_st, [_vp, _sz, _vp, _sz, _f, _vp, _vp]),
+    "pbf_k_resolve_velocities": (_st, [_vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _sz,
+                                       _f, _f, _f, _vp]),
+    "pbf_k_poly6": (_st, [_vp, _sz, _f, _vp, _vp]),
+    "pbf_k_spiky_gradient": (_st, [_vp, _sz, _f, _vp, _vp]),
+    "pbf_k_artificial_pressure": (_st, [_vp, _sz, _f, _f, _i, _f, _vp, _vp]),
+    "pbf_grid_create": (_st, [C.POINTER(_vp)]),
+    "pbf_grid_destroy": (_st, [_vp]),
+    "pbf_grid_initialize": (_st, [_vp, _sz, float3, float3, _f]),
+    "pbf_grid_build": (_st, [_vp, _vp, _sz]),
+    "pbf_grid_sorted_keys": (_vp, [_vp]),
+    "pbf_grid_sorted_indices": (_vp, [_vp]),
+    "pbf_grid_cell_start": (_vp, [_vp]),
+    "pbf_grid_cell_end": (_vp, [_vp]),
+    "pbf_grid_info": (_st, [_vp, C.POINTER(int3), C.POINTER(_sz), C.POINTER(_sz),
+                            C.POINTER(_f), C.POINTER(float3), C.POINTER(float3)]),
+    "pbf_collision_create": (_st, [C.POINTER(_vp)]),
+    "pbf_collision_destroy": (_st, [_vp]),
+    "pbf_collision_set_container": (_st, [_vp, Container]),
+    "pbf_collision_set_spheres": (_st, [_vp, _vp, _sz]),
+    "pbf_collision_clear_spheres": (_st, [_vp]),
+    "pbf_collision_set_boxes": (_st, [_vp, _vp, _sz]),
+    "pbf_collision_clear_boxes": (_st, [_vp]),
+    "pbf_collision_set_planes": (_st, [_vp, _vp, _sz]),
+    "pbf_collision_clear_planes": (_st, [_vp]),
+    "pbf_collision_solve": (_st, [_vp, _vp, _sz, _f]),
+    "pbf_collision_resolve_velocities": (_st, [_vp, _vp, _vp, _vp, _sz, _f, _f, _f]),
+    "pbf_solver_create": (_st, [C.POINTER(_vp)]),
+    "pbf_solver_destroy": (_st, [_vp]),
+    "pbf_solver_initialize": (_st, [_vp, _sz, float3, float3, C.POINTER(SimulationParams)]),
+    "pbf_solver_set_particles": (_st, [_vp, _vp, _vp, _sz]),
+    "pbf_solver_set_particles_device": (_st, [_vp, _vp, _vp, _sz]),
+    "pbf_solver_set_spheres": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_set_boxes": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_set_planes": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_step": (_st, [_vp]),
+    "pbf_solver_run_one": (_st, [_vp]),
+    "pbf_solver_run": (_st, [_vp, _sz]),
+    "pbf_solver_copy_positions_to_host": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_copy_velocities_to_host": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_copy_positions_to_device": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_copy_velocities_to_device": (_st, [_vp, _vp, _sz]),
+    "pbf_solver_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
+    "pbf_solver_stream": (_vp, [_vp]),
+    "pbf_solver_set_profiling": (_st, [_vp, _i]),
+    "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
+}
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libpbf_amd.so; raises if the HIP library was not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python fluid-simulator_amd/build.py or __graft_entry__.build())")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def declared_symbols() -> list[str]:
+    return sorted(_SIGNATURES)
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = lib().pbf_last_error()
+        raise _ERRORS.get(status, DeviceError)(msg.decode() if msg else f"status {status}")
+
+
+def _ptr(a):
+    """Device pointer of a DeviceArray / int, or host pointer of a numpy array."""
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    if isinstance(a, int):
+        return a
+    raise TypeError(f"cannot take a pointer of {type(a)}")
+
+
+def as_float4(a, n=None) -> np.ndarray:
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if arr.ndim == 1:
+        arr = arr.reshape(-1, 4)
+    if arr.shape[-1] != 4:
+        raise InvalidArgument("float4 arrays must have shape (n, 4)")
+    return arr
+
+
+# --------------------------------------------------------------------------
+# Device memory
+# --------------------------------------------------------------------------
+class DeviceArray:
+    """An owned device allocation (CudaBuffer<T>, include/pbf/cuda_buffer.hpp)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(lib().pbf_device_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, min_bytes: int = 0) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(max(a.nbytes, min_bytes, 4))
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise OutOfRange("Source is larger than device buffer")
+        _check(lib().pbf_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        if not out.flags.c_contiguous:
+            raise InvalidArgument("destination must be contiguous")
+        if out.nbytes > self.nbytes:
+            raise OutOfRange("Copy exceeds device buffer size")
+        _check(lib().pbf_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def to_numpy(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        return self.download(out)
+
+    def fill_bytes(self, value: int) -> None:
+        _check(lib().pbf_memset(self.ptr, int(value), self.nbytes))
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().pbf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _check(lib().pbf_device_synchronize())
+
+
+def read_device(ptr: int, dtype, shape) -> np.ndarray:
+    out = np.empty(shape, dtype=dtype)
+    if out.nbytes:
+        _check(lib().pbf_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes))
+    return out
+
+
+# --------------------------------------------------------------------------
+# SpatialGrid
+# --------------------------------------------------------------------------
+class SpatialGrid:
+    """SpatialGrid (include/pbf/spatial_grid.cuh:11-62)."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().pbf_grid_create(C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pbf_grid_destroy(self._h)
+            self._h = None
+
+    def initialize(self, maxParticles, minBounds, maxBounds, cellSize):
+        _check(lib().pbf_grid_initialize(self._h, int(maxParticles), _f3(minBounds),
+                                         _f3(maxBounds), float(cellSize)))
+
+    def build(self, predictedPositions, particleCount):
+        _check(lib().pbf_grid_build(self._h, _ptr(predictedPositions), int(particleCount)))
+
+    def _info(self):
+        g, nc, mp, cs, mn, mx = int3(), _sz(), _sz(), _f(), float3(), float3()
+        _check(lib().pbf_grid_info(self._h, C.byref(g), C.byref(nc), C.byref(mp), C.byref(cs),
+                                   C.byref(mn), C.byref(mx)))
+        return g, nc.value, mp.value, cs.value, mn, mx
+
+    def sortedKeys(self):
+        return lib().pbf_grid_sorted_keys(self._h)
+
+    def sortedIndices(self):
+        return lib().pbf_grid_sorted_indices(self._h)
+
+    def cellStart(self):
+        return lib().pbf_grid_cell_start(self._h)
+
+    def cellEnd(self):
+        return lib().pbf_grid_cell_end(self._h)
+
+    def gridSize(self):
+        return self._info()[0]
+
+    def numCells(self):
+        return self._info()[1]
+
+    def maxParticles(self):
+        return self._info()[2]
+
+    def cellSize(self):
+        return self._info()[3]
+
+    def minBounds(self):
+        return self._info()[4]
+
+    def maxBounds(self):
+        return self._info()[5]
+
+    def read(self, n):
+        """(sortedKeys, sortedIndices, cellStart, cellEnd) as numpy arrays."""
+        cells = self.numCells()
+        return (read_device(self.sortedKeys(), np.uint32, (n,)),
+                read_device(self.sortedIndices(), np.uint32, (n,)),
+                read_device(self.cellStart(), np.int32, (cells,)),
+                read_device(self.cellEnd(), np.int32, (cells,)))
+
+
+# --------------------------------------------------------------------------
+# CollisionSystem
+# --------------------------------------------------------------------------
+def _struct_array(items, conv, ctype):
+    items = [conv(x) for x in items]
+    arr = (ctype * max(1, len(items)))(*items) if items else None
+    return arr, len(items)
+
+
+class CollisionSystem:
+    """CollisionSystem (include/pbf/collision_system.hpp:11-57)."""
+
+    def __init__(self, container=None):
+        h = _vp()
+        _check(lib().pbf_collision_create(C.byref(h)))
+        self._h = h.value
+        if container is not None:
+            self.setContainer(container)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pbf_collision_destroy(self._h)
+            self._h = None
+
+    def setContainer(self, container):
+        if not isinstance(container, Container):
+            mn, mx = container
+            container = Container(_f3(mn), _f3(mx))
+        _check(lib().pbf_collision_set_container(self._h, container))
+
+    def setSpheres(self, spheres):
+        arr, n = _struct_array(spheres, _sphere, SphereCollider)
+        _check(lib().pbf_collision_set_spheres(self._h, arr, n))
+
+    def clearSpheres(self):
+        _check(lib().pbf_collision_clear_spheres(self._h))
+
+    def setBoxes(self, boxes):
+        arr, n = _struct_array(boxes, _box, BoxCollider)
+        _check(lib().pbf_collision_set_boxes(self._h, arr, n))
+
+    def clearBoxes(self):
+        _check(lib().pbf_collision_clear_boxes(self._h))
+
+    def setPlanes(self, planes):
+        arr, n = _struct_array(planes, _plane, PlaneCollider)
+        _check(lib().pbf_collision_set_planes(self._h, arr, n))
+
+    def clearPlanes(self):
+        _check(lib().pbf_collision_clear_planes(self._h))
+
+    def solve(self, predictedPositions, particleCount, particleRadius):
+        _check(lib().pbf_collision_solve(self._h, _ptr(predictedPositions), int(particleCount),
+                                         float(particleRadius)))
+
+    def resolveVelocities(self, positions, *args):
+        """resolveVelocities(positions, velocities, n, r, e, f) or
+        resolveVelocities(positions, incoming, velocities, n, r, e, f)."""
+        if len(args) == 5:
+            velocities, n, r, e, f = args
+            incoming = velocities
+        else:
+            incoming, velocities, n, r, e, f = args
+        _check(lib().pbf_collision_resolve_velocities(self._h, _ptr(positions), _ptr(incoming),
+                                                      _ptr(velocities), int(n), float(r),
+                                                      float(e), float(f)))
+
+
+# --------------------------------------------------------------------------
+# PBFSolver
+# --------------------------------------------------------------------------
+class PBFSolver:
+    """PBFSolver (include/pbf/pbf_solver.hpp:13-70)."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().pbf_solver_create(C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pbf_solver_destroy(self._h)
+            self._h = None
+
+    def initialize(self, maxParticles, minBounds, maxBounds, params: SimulationParams):
+        _check(lib().pbf_solver_initialize(self._h, int(maxParticles), _f3(minBounds),
+                                           _f3(maxBounds), C.byref(params)))
+
+    def setParticles(self, positions, velocities, particleCount=None):
+        pos = None if positions is None else as_float4(positions)
+        vel = None if velocities is None else as_float4(velocities)
+        n = particleCount if particleCount is not None else (0 if pos is None else len(pos))
+        _check(lib().pbf_solver_set_particles(self._h, _ptr(pos), _ptr(vel), int(n)))
+
+    def setParticlesDevice(self, positions, velocities, particleCount):
+        _check(lib().pbf_solver_set_particles_device(self._h, _ptr(positions), _ptr(velocities),
+                                                     int(particleCount)))
+
+    def setSpheres(self, spheres):
+        arr, n = _struct_array(spheres, _sphere, SphereCollider)
+        _check(lib().pbf_solver_set_spheres(self._h, arr, n))
+
+    def setBoxes(self, boxes):
+        arr, n = _struct_array(boxes, _box, BoxCollider)
+        _check(lib().pbf_solver_set_boxes(self._h, arr, n))
+
+    def setPlanes(self, planes):
+        arr, n = _struct_array(planes, _plane, PlaneCollider)
+        _check(lib().pbf_solver_set_planes(self._h, arr, n))
+
+    def step(self):
+        _check(lib().pbf_solver_step(self._h))
+
+    def run(self, frameCount=None):
+        if frameCount is None:
+            _check(lib().pbf_solver_run_one(self._h))
+        else:
+            _check(lib().pbf_solver_run(self._h, int(frameCount)))
+
+    def _copy(self, fn, out, particleCount):
+        n = int(particleCount) if particleCount is not None else self.particleCount()
+        if out is None and n != 0 and particleCount is not None and particleCount < 0:
+            raise InvalidArgument("negative count")
+        if out is None:
+            out_arr = np.zeros((n, 4), dtype=np.float32) if n else None
+        else:
+            out_arr = out
+        _check(fn(self._h, _ptr(out_arr), n))
+        return out_arr
+
+    def copyPositionsToHost(self, out=None, particleCount=None):
+        return self._copy(lib().pbf_solver_copy_positions_to_host, out, particleCount)
+
+    def copyVelocitiesToHost(self, out=None, particleCount=None):
+        return self._copy(lib().pbf_solver_copy_velocities_to_host, out, particleCount)
+
+    def copyPositionsToDevice(self, out: "DeviceArray", particleCount):
+        _check(lib().pbf_solver_copy_positions_to_device(self._h, _ptr(out), int(particleCount)))
+
+    def copyVelocitiesToDevice(self, out: "DeviceArray", particleCount):
+        _check(lib().pbf_solver_copy_velocities_to_device(self._h, _ptr(out), int(particleCount)))
+
+    def particleCount(self) -> int:
+        n, m = _sz(), _sz()
+        _check(lib().pbf_solver_counts(self._h, C.byref(n), C.byref(m)))
+        return n.value
+
+    def maxParticles(self) -> int:
+        n, m = _sz(), _sz()
+        _check(lib().pbf_solver_counts(self._h, C.byref(n), C.byref(m)))
+        return m.value
+
+    def stream(self) -> int:
+        return lib().pbf_solver_stream(self._h) or 0
+
+    def setProfiling(self, enabled: bool):
+        _check(lib().pbf_solver_set_profiling(self._h, 1 if enabled else 0))
+
+    def kernelTimes(self) -> dict:
+        cap = 64
+        names = C.create_string_buffer(8192)
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        count = _sz()
+        _check(lib().pbf_solver_kernel_times(self._h, names, 8192, ms, launches, cap,
+                                             C.byref(count)))
+        keys = names.value.decode().split("\n")[:count.value]
+        return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
+
+
+from . import kernels  # noqa: E402  (device kernel wrappers need the classes above)

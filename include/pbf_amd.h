@@ -1,0 +1,370 @@
+/*
+ * pbf_amd.h -- C ABI of the MI355X (gfx950) Position Based Fluids library.
+ *
+ * The library replaces the CUDA library `pbf` of DusanCojic/Fluid-Simulator.
+ * Every entry point below names the reference interface it stands in for.
+ * The ABI is plain C: POD structs, raw device pointers, sizes, an opaque
+ * stream handle and integer status codes.  No HIP, CUDA or torch type
+ * appears in a signature.
+ *
+ * Conventions
+ *  - Functions return a pbf_status.  On failure, pbf_last_error() returns a
+ *    thread-local message.  The reference throws C++ exceptions; the status
+ *    codes map one-to-one onto its exception types (see pbf_status).
+ *  - Pointers named d_* or documented as "device" are device (HBM) pointers.
+ *    Pointers named h_* are host pointers.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the legacy default
+ *    stream, which is what the reference launches on.
+ *  - Kernel launchers (pbf_k_*) are asynchronous: they return after the
+ *    launch, like a CUDA `<<<>>>` launch followed by cudaGetLastError().
+ *  - Object handles (pbf_grid, pbf_collision, pbf_solver) own device memory
+ *    and synchronise where the reference does.
+ */
+#ifndef PBF_AMD_H
+#define PBF_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Plain data types.  Byte layouts equal CUDA's float3/float4/int3 and the    */
+/* reference PODs, so buffers can be shared with reference-shaped callers.   */
+/* ------------------------------------------------------------------------ */
+
+/* CUDA float3: 12 bytes, 4-byte aligned. */
+typedef struct pbf_float3 { float x, y, z; } pbf_float3;
+
+/* CUDA float4: 16 bytes, 16-byte aligned.  .w is carried, never interpreted. */
+typedef struct pbf_float4 { float x, y, z, w; } __attribute__((aligned(16))) pbf_float4;
+
+/* CUDA int3: 12 bytes, 4-byte aligned. */
+typedef struct pbf_int3 { int x, y, z; } pbf_int3;
+
+/* Replaces SimulationParams (include/pbf/simulation_params.hpp:6-29).
+ * 72 bytes; the field order and offsets are those of the reference. */
+typedef struct pbf_sim_params {
+    float dt;                   /*  0: frame step; a substep is dt / substeps */
+    float restDensity;          /*  4 */
+    float particleMass;         /*  8 */
+    float particleRadius;       /* 12: collision radius */
+    float collisionRestitution; /* 16 */
+    float collisionFriction;    /* 20 */
+    float smoothingRadius;      /* 24: SPH kernel radius h */
+    float lambdaEpsilon;        /* 28: constraint relaxation */
+    int   solverIterations;     /* 32 */
+    int   substeps;             /* 36 */
+    pbf_float3 gravity;         /* 40 */
+    float scorrK;               /* 52: artificial pressure strength */
+    int   scorrN;               /* 56: artificial pressure exponent */
+    float scorrDeltaQ;          /* 60: artificial pressure reference distance */
+    float xsphViscosity;        /* 64 */
+    float vorticityStrength;    /* 68 */
+} pbf_sim_params;
+
+/* Replace Container / SphereCollider / BoxCollider / PlaneCollider
+ * (include/pbf/colliders.hpp:7-25). */
+typedef struct pbf_container { pbf_float3 min, max; } pbf_container;          /* 24 B */
+typedef struct pbf_sphere { pbf_float3 center; float radius; } pbf_sphere;    /* 16 B */
+typedef struct pbf_box { pbf_float3 center, halfExtents; } pbf_box;          /* 24 B */
+typedef struct pbf_plane { pbf_float3 point, normal; } pbf_plane;            /* 24 B */
+
+/* Status codes.  The right-hand column is the exception the reference
+ * throws for the same condition. */
+typedef enum pbf_status {
+    PBF_OK = 0,
+    PBF_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+    PBF_ERR_LOGIC = 2,            /* std::logic_error (use before initialize) */
+    PBF_ERR_OUT_OF_RANGE = 3,     /* std::out_of_range */
+    PBF_ERR_LENGTH = 4,           /* std::length_error */
+    PBF_ERR_OVERFLOW = 5,         /* std::overflow_error (neighbour capacity) */
+    PBF_ERR_NONCONVERGENCE = 6,   /* std::runtime_error (collision passes) */
+    PBF_ERR_DEVICE = 7            /* std::runtime_error (HIP runtime error) */
+} pbf_status;
+
+/* Bits of the device status word the fused step writes instead of the
+ * reference's per-phase host synchronisations. */
+#define PBF_STATUS_NEIGHBOR_OVERFLOW 0x1u
+#define PBF_STATUS_COLLISION_NONCONVERGENCE 0x2u
+
+/* Thread-local message for the last failing call on this thread. */
+const char* pbf_last_error(void);
+
+/* Library version string ("pbf_amd <version> gfx950"). */
+const char* pbf_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory helpers (stand in for CudaBuffer<T>,                         */
+/* include/pbf/cuda_buffer.hpp:43-113).                                       */
+/* ------------------------------------------------------------------------ */
+pbf_status pbf_device_alloc(void** d_ptr, size_t bytes);
+pbf_status pbf_device_free(void* d_ptr);
+pbf_status pbf_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+pbf_status pbf_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+pbf_status pbf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+/* Memset semantics: every byte gets the low 8 bits of byteValue
+ * (CudaBuffer::fillBytes, cuda_buffer.hpp:67-77). */
+pbf_status pbf_memset(void* d_dst, int byteValue, size_t bytes);
+pbf_status pbf_device_synchronize(void);
+pbf_status pbf_stream_create(void** stream);
+pbf_status pbf_stream_destroy(void* stream);
+pbf_status pbf_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel launchers.  Each replaces one __global__ kernel of the reference   */
+/* and keeps its argument meaning.  Launches are asynchronous on `stream`.   */
+/* ------------------------------------------------------------------------ */
+
+/* integrate (include/pbf/integration.cuh:6-8, src/pbf/integration.cu:6-32):
+ * v += g*dt; p* = x + v*dt; p*.w = x.w. */
+pbf_status pbf_k_integrate(const pbf_float4* d_position, pbf_float4* d_predicted,
+                           pbf_float4* d_velocity, int particleCount, float dt,
+                           pbf_float3 gravity, void* stream);
+
+/* findNeighbors (include/pbf/neighbors.cuh:16-23, src/pbf/neighbors.cu:3-91).
+ * Slot-major list: d_neighbors[k * particleStride + i].  d_count receives the
+ * full count, which may exceed maxNeighbors; then *d_overflowFlag = 1 if the
+ * flag pointer is not NULL. */
+pbf_status pbf_k_find_neighbors(const pbf_float4* d_predicted, const int* d_cellStart,
+                                const int* d_cellEnd, pbf_int3 gridSize,
+                                pbf_float3 minBounds, float cellSize,
+                                size_t particleCount, float smoothingRadius,
+                                uint32_t* d_neighbors, int* d_count, int maxNeighbors,
+                                size_t particleStride, int* d_overflowFlag, void* stream);
+
+/* computeLambda (include/pbf/pbf_solver_kernels.cuh:16-21,
+ * src/pbf/pbf_solver_kernels.cu:26-105). */
+pbf_status pbf_k_compute_lambda(const pbf_float4* d_predicted, const uint32_t* d_neighbors,
+                                const int* d_count, int maxNeighbors, size_t particleCount,
+                                float smoothingRadius, float particleMass,
+                                float restDensity, float epsilon, float* d_lambda,
+                                size_t neighborParticleStride, void* stream);
+
+/* computeDeltaPosition (include/pbf/pbf_solver_kernels.cuh:49-53,
+ * src/pbf/pbf_solver_kernels.cu:108-182), with computeArtificialPressure
+ * (include/pbf/pbf_solver_kernels.cuh:24-47). */
+pbf_status pbf_k_compute_delta_position(const pbf_float4* d_predicted,
+                                        const uint32_t* d_neighbors, const int* d_count,
+                                        int maxNeighbors, const float* d_lambda,
+                                        size_t particleCount, float smoothingRadius,
+                                        float particleMass, float restDensity, float scorrK,
+                                        int scorrN, float scorrDeltaQ, pbf_float4* d_delta,
+                                        size_t neighborParticleStride, void* stream);
+
+/* applyDeltaPosition (src/pbf/pbf_solver_kernels.cu:184-193). */
+pbf_status pbf_k_apply_delta_position(pbf_float4* d_predicted, const pbf_float4* d_delta,
+                                      size_t particleCount, void* stream);
+
+/* updateVelocityAndPosition (src/pbf/pbf_solver_kernels.cu:195-211). */
+pbf_status pbf_k_update_velocity_and_position(pbf_float4* d_position,
+                                              const pbf_float4* d_predicted,
+                                              pbf_float4* d_velocity, size_t particleCount,
+                                              float inverseDt, void* stream);
+
+/* applyXsphViscosity (include/pbf/pbf_solver_kernels.cuh:63-70,
+ * src/pbf/pbf_solver_kernels.cu:213-266).  Jacobi: reads d_vin, writes d_vout. */
+pbf_status pbf_k_apply_xsph_viscosity(const pbf_float4* d_predicted,
+                                      const uint32_t* d_neighbors, const int* d_count,
+                                      int maxNeighbors, const pbf_float4* d_vin,
+                                      pbf_float4* d_vout, size_t particleCount,
+                                      float smoothingRadius, float xsphViscosity,
+                                      size_t neighborParticleStride, void* stream);
+
+/* computeVorticity (include/pbf/pbf_solver_kernels.cuh:72-79,
+ * src/pbf/pbf_solver_kernels.cu:268-315). */
+pbf_status pbf_k_compute_vorticity(const pbf_float4* d_position, const pbf_float4* d_velocity,
+                                   const uint32_t* d_neighbors, const int* d_count,
+                                   int maxNeighbors, size_t particleCount,
+                                   float smoothingRadius, pbf_float4* d_vorticity,
+                                   size_t neighborParticleStride, void* stream);
+
+/* applyVorticityConfinement (include/pbf/pbf_solver_kernels.cuh:81-93,
+ * src/pbf/pbf_solver_kernels.cu:317-385). */
+pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* d_position,
+                                             const uint32_t* d_neighbors, const int* d_count,
+                                             int maxNeighbors, const pbf_float4* d_vorticity,
+                                             const pbf_float4* d_vin, pbf_float4* d_vout,
+                                             size_t particleCount, float smoothingRadius,
+                                             float dt, float vorticityStrength,
+                                             size_t neighborParticleStride, void* stream);
+
+/* gatherSolverState (include/pbf/pbf_solver_kernels.cuh:95-105,
+ * src/pbf/pbf_solver_kernels.cu:387-407). */
+pbf_status pbf_k_gather_solver_state(const uint32_t* d_sortedIndices,
+                                     const pbf_float4* d_position, const pbf_float4* d_predicted,
+                                     const pbf_float4* d_velocity,
+                                     const pbf_float4* d_collisionInputVelocity,
+                                     const uint32_t* d_stableIds, pbf_float4* d_sortedPosition,
+                                     pbf_float4* d_sortedPredicted, pbf_float4* d_sortedVelocity,
+                                     pbf_float4* d_sortedCollisionInputVelocity,
+                                     uint32_t* d_sortedStableIds, size_t particleCount,
+                                     void* stream);
+
+/* initializeStableParticleIds (src/pbf/pbf_solver_kernels.cu:409-416). */
+pbf_status pbf_k_initialize_stable_particle_ids(uint32_t* d_ids, size_t particleCount,
+                                                void* stream);
+
+/* scatterFloat4ByStableId (src/pbf/pbf_solver_kernels.cu:418-427). */
+pbf_status pbf_k_scatter_float4_by_stable_id(const pbf_float4* d_values, const uint32_t* d_ids,
+                                             pbf_float4* d_valuesInOriginalOrder,
+                                             size_t particleCount, void* stream);
+
+/* Collision kernels (include/pbf/collision_kernels.cuh:8-34,
+ * src/pbf/collision_kernels.cu:47-371).  Collider pointers are device
+ * pointers; d_correctionFlag may be NULL. */
+pbf_status pbf_k_solve_container(pbf_float4* d_predicted, size_t particleCount,
+                                 const pbf_container* d_container, float particleRadius,
+                                 int* d_correctionFlag, void* stream);
+pbf_status pbf_k_solve_spheres(pbf_float4* d_predicted, size_t particleCount,
+                               const pbf_sphere* d_spheres, size_t sphereCount,
+                               float particleRadius, int* d_correctionFlag, void* stream);
+pbf_status pbf_k_solve_boxes(pbf_float4* d_predicted, size_t particleCount,
+                             const pbf_box* d_boxes, size_t boxCount, float particleRadius,
+                             int* d_correctionFlag, void* stream);
+pbf_status pbf_k_solve_planes(pbf_float4* d_predicted, size_t particleCount,
+                              const pbf_plane* d_planes, size_t planeCount,
+                              float particleRadius, int* d_correctionFlag, void* stream);
+pbf_status pbf_k_resolve_velocities(const pbf_float4* d_position,
+                                    const pbf_float4* d_incomingVelocity,
+                                    pbf_float4* d_velocity, size_t particleCount,
+                                    const pbf_container* d_container,
+                                    const pbf_sphere* d_spheres, size_t sphereCount,
+                                    const pbf_box* d_boxes, size_t boxCount,
+                                    const pbf_plane* d_planes, size_t planeCount,
+                                    float particleRadius, float restitution, float friction,
+                                    void* stream);
+
+/* Device evaluation of the inline SPH helpers, batched (the reference tests
+ * reach them through single-thread test kernels, test/test_sph_helpers.cu:10-18
+ * and test/test_pbf_solver_kernels.cu:14-26):
+ * poly6 (include/pbf/sph_helpers.cuh:38-46), spikyGradient (:85-95),
+ * computeArtificialPressure (include/pbf/pbf_solver_kernels.cuh:24-47). */
+pbf_status pbf_k_poly6(const pbf_float3* d_displacement, size_t count, float smoothingRadius,
+                       float* d_out, void* stream);
+pbf_status pbf_k_spiky_gradient(const pbf_float3* d_displacement, size_t count,
+                                float smoothingRadius, pbf_float3* d_out, void* stream);
+pbf_status pbf_k_artificial_pressure(const float* d_distanceSquared, size_t count,
+                                     float smoothingRadius, float scorrK, int scorrN,
+                                     float scorrDeltaQ, float* d_out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* SpatialGrid (include/pbf/spatial_grid.cuh:11-62, src/pbf/spatial_grid.cu). */
+/* Cell size, bounds and the 32-bit key layout x + y*gx + z*gx*gy are the    */
+/* reference's; the sort is this library's LDS-staged LSD radix sort over    */
+/* ceil(log2(numCells)) key bits, stable in the input slot.                  */
+/* ------------------------------------------------------------------------ */
+typedef struct pbf_grid pbf_grid;
+
+pbf_status pbf_grid_create(pbf_grid** grid);
+pbf_status pbf_grid_destroy(pbf_grid* grid);
+/* SpatialGrid::initialize (spatial_grid.cu:10-97).  Strong guarantee. */
+pbf_status pbf_grid_initialize(pbf_grid* grid, size_t maxParticles, pbf_float3 minBounds,
+                               pbf_float3 maxBounds, float cellSize);
+/* SpatialGrid::build (spatial_grid.cu:139-226): keys, stable sort, ranges.
+ * Synchronises before returning, like the reference. */
+pbf_status pbf_grid_build(pbf_grid* grid, const pbf_float4* d_predicted, size_t particleCount);
+/* Device-pointer getters (spatial_grid.cu:230-244); valid until the next
+ * initialize/build. */
+const uint32_t* pbf_grid_sorted_keys(const pbf_grid* grid);
+const uint32_t* pbf_grid_sorted_indices(const pbf_grid* grid);
+const int* pbf_grid_cell_start(const pbf_grid* grid);
+const int* pbf_grid_cell_end(const pbf_grid* grid);
+/* Value getters (spatial_grid.cu:246-268).  Any out pointer may be NULL. */
+pbf_status pbf_grid_info(const pbf_grid* grid, pbf_int3* gridSize, size_t* numCells,
+                         size_t* maxParticles, float* cellSize, pbf_float3* minBounds,
+                         pbf_float3* maxBounds);
+
+/* ------------------------------------------------------------------------ */
+/* CollisionSystem (include/pbf/collision_system.hpp:11-57,                  */
+/* src/pbf/collision_system.cu:29-277).                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct pbf_collision pbf_collision;
+
+pbf_status pbf_collision_create(pbf_collision** cs);
+pbf_status pbf_collision_destroy(pbf_collision* cs);
+pbf_status pbf_collision_set_container(pbf_collision* cs, pbf_container container);
+/* Host arrays; count 0 clears (collision_system.cu:52-131). */
+pbf_status pbf_collision_set_spheres(pbf_collision* cs, const pbf_sphere* h_spheres, size_t count);
+pbf_status pbf_collision_clear_spheres(pbf_collision* cs);
+pbf_status pbf_collision_set_boxes(pbf_collision* cs, const pbf_box* h_boxes, size_t count);
+pbf_status pbf_collision_clear_boxes(pbf_collision* cs);
+pbf_status pbf_collision_set_planes(pbf_collision* cs, const pbf_plane* h_planes, size_t count);
+pbf_status pbf_collision_clear_planes(pbf_collision* cs);
+/* CollisionSystem::solve (collision_system.cu:133-172): projection to a fixed
+ * point, at most 32 passes; PBF_ERR_NONCONVERGENCE otherwise.  Synchronous. */
+pbf_status pbf_collision_solve(pbf_collision* cs, pbf_float4* d_predicted, size_t particleCount,
+                               float particleRadius);
+/* CollisionSystem::resolveVelocities, both overloads (collision_system.cu:174-218);
+ * pass d_incomingVelocity == d_velocity for the two-pointer form. */
+pbf_status pbf_collision_resolve_velocities(pbf_collision* cs, const pbf_float4* d_position,
+                                            const pbf_float4* d_incomingVelocity,
+                                            pbf_float4* d_velocity, size_t particleCount,
+                                            float particleRadius, float restitution,
+                                            float friction);
+
+/* ------------------------------------------------------------------------ */
+/* PBFSolver (include/pbf/pbf_solver.hpp:13-70, src/pbf/pbf_solver.cu).       */
+/* ------------------------------------------------------------------------ */
+typedef struct pbf_solver pbf_solver;
+
+pbf_status pbf_solver_create(pbf_solver** solver);
+pbf_status pbf_solver_destroy(pbf_solver* solver);
+/* PBFSolver::initialize (pbf_solver.cu:23-116). */
+pbf_status pbf_solver_initialize(pbf_solver* solver, size_t maxParticles, pbf_float3 minBounds,
+                                 pbf_float3 maxBounds, const pbf_sim_params* params);
+/* PBFSolver::setParticles (pbf_solver.cu:118-152): host arrays, validated
+ * before any upload. */
+pbf_status pbf_solver_set_particles(pbf_solver* solver, const pbf_float4* h_positions,
+                                    const pbf_float4* h_velocities, size_t particleCount);
+/* Same contract with device arrays (no host validation of values; the
+ * caller guarantees finiteness).  Used when inputs are already in HBM. */
+pbf_status pbf_solver_set_particles_device(pbf_solver* solver, const pbf_float4* d_positions,
+                                           const pbf_float4* d_velocities, size_t particleCount);
+/* PBFSolver::setSpheres/setBoxes/setPlanes (pbf_solver.cu:174-184). */
+pbf_status pbf_solver_set_spheres(pbf_solver* solver, const pbf_sphere* h_spheres, size_t count);
+pbf_status pbf_solver_set_boxes(pbf_solver* solver, const pbf_box* h_boxes, size_t count);
+pbf_status pbf_solver_set_planes(pbf_solver* solver, const pbf_plane* h_planes, size_t count);
+/* PBFSolver::step (pbf_solver.cu:186-328): one frame of params.substeps
+ * substeps.  Returns after the device finished, like the reference.
+ * Overflow and non-convergence are detected on the device and reported here
+ * as PBF_ERR_OVERFLOW / PBF_ERR_NONCONVERGENCE. */
+pbf_status pbf_solver_step(pbf_solver* solver);
+/* PBFSolver::run() (pbf_solver.cu:330-335). */
+pbf_status pbf_solver_run_one(pbf_solver* solver);
+/* PBFSolver::run(frameCount) (pbf_solver.cu:337-343).  Frames are enqueued
+ * back to back and the status word is read once at the end. */
+pbf_status pbf_solver_run(pbf_solver* solver, size_t frameCount);
+/* PBFSolver::copyPositionsToHost / copyVelocitiesToHost (pbf_solver.cu:345-379):
+ * API (stable id) order. */
+pbf_status pbf_solver_copy_positions_to_host(pbf_solver* solver, pbf_float4* h_out,
+                                             size_t particleCount);
+pbf_status pbf_solver_copy_velocities_to_host(pbf_solver* solver, pbf_float4* h_out,
+                                              size_t particleCount);
+/* Device-output variants of the two copies (API order into d_out). */
+pbf_status pbf_solver_copy_positions_to_device(pbf_solver* solver, pbf_float4* d_out,
+                                               size_t particleCount);
+pbf_status pbf_solver_copy_velocities_to_device(pbf_solver* solver, pbf_float4* d_out,
+                                                size_t particleCount);
+/* Active particle count and capacity. */
+pbf_status pbf_solver_counts(const pbf_solver* solver, size_t* particleCount,
+                             size_t* maxParticles);
+/* The stream the solver enqueues on (hipStream_t as void*). */
+void* pbf_solver_stream(pbf_solver* solver);
+
+/* Per-kernel timing on the solver's stream with HIP events.  While enabled,
+ * every launch of the step is bracketed by events; pbf_solver_kernel_times
+ * returns accumulated milliseconds and launch counts per kernel name.
+ * Names are written as one '\n'-separated string. */
+pbf_status pbf_solver_set_profiling(pbf_solver* solver, int enabled);
+pbf_status pbf_solver_kernel_times(pbf_solver* solver, char* names, size_t namesBytes,
+                                   double* milliseconds, uint64_t* launches, size_t capacity,
+                                   size_t* kernelCount);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBF_AMD_H */
