@@ -1,9 +1,11 @@
 """Throughput of the PBF step on one MI355X.
 
-Workload: the seeded dam break at 1,048,576 particles (128^3 block, h = 0.1,
-rest density 6378, 4 constraint iterations, dt = 1/60 s, XSPH, vorticity
-confinement, s_corr, container collisions); state resident in HBM before the
-timed region.  A "step" is one PBFSolver::step() (substeps = 1).
+Workload: the seeded dam break at 1,048,576 particles (128 x 64 x 128 block,
+h = 0.1, rest density 6378, 4 constraint iterations, 1/60 s frames of 2
+substeps, XSPH, vorticity confinement, s_corr, container collisions); state
+resident in HBM before the timed region.  The throughput unit is the
+particle-step = one particle advanced by one PBF substep; --steps counts
+PBFSolver::step() calls (frames).
 
 Prints one JSON line:
   value          particle-steps/s over exactly --steps timed steps
@@ -73,14 +75,16 @@ def cpu_baseline(scene, seconds_budget: float = 25.0) -> dict:
         s.step()
         times.append(time.perf_counter() - t)
     med = statistics.median(times)
+    sub = scene.params.substeps
     return {
-        "value": scene.n / med,
+        "value": scene.n * sub / med,
         "unit": "particle-steps/s",
         "cores": threads(),
         "kind": "port",
-        "sample": f"{scene.name}: {scene.n} particles, 1 warm-up step then median of {reps} "
-                  f"timed steps of the FP32 OpenMP restatement (5 grid builds per step, "
-                  f"slot-major neighbour lists, one loop per reference kernel)",
+        "sample": f"{scene.name}: {scene.n} particles, 1 warm-up frame then median of {reps} "
+                  f"timed frames ({sub} substeps each) of the FP32 OpenMP restatement "
+                  f"(5 grid builds per substep, slot-major neighbour lists, one loop per "
+                  f"reference kernel)",
     }
 
 
@@ -108,13 +112,17 @@ def main() -> int:
     solver.initialize(n, scene.minBounds, scene.maxBounds, scene.params)
     solver.setParticles(scene.positions, scene.velocities)
 
+    # One timed "step" is one PBFSolver::step() call, i.e. one frame of
+    # `substeps` PBF substeps; throughput counts particle-substeps (the
+    # reference benchmark's particle-substeps/s, pbf_benchmark.cu:201-211).
+    substeps = scene.params.substeps
     solver.run(a.warmup)
     pbf_amd.synchronize()
     t0 = time.perf_counter()
     solver.run(a.steps)
     pbf_amd.synchronize()
     elapsed = time.perf_counter() - t0
-    value = n * a.steps / elapsed
+    value = n * a.steps * substeps / elapsed
 
     # Per-kernel launch times (HIP events on the solver's stream), after the
     # timed region so the events do not perturb it.
@@ -155,6 +163,9 @@ def main() -> int:
 
     out = {
         "metric": "particle-steps/s @1M (4 iters) + % HBM roofline",
+        "step_unit": "one PBF substep of one particle (integrate, 4 constraint iterations, "
+                     "XSPH, vorticity, contacts); a timed step() call is one frame of "
+                     f"{substeps} substeps",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -166,10 +177,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{scene.name}: {n} particles, 4 iterations, substeps 1, "
+        "config": {"workload": f"{scene.name}: {n} particles, 4 iterations, "
+                               f"{substeps} substeps per 1/60 s frame, "
                                "XSPH + vorticity + s_corr + container",
                    "particles": n, "iterations": scene.params.solverIterations,
-                   "parallelism": "single-gpu"},
+                   "substeps": substeps, "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernel_rows,

@@ -1,0 +1,110 @@
+"""Independent FP64 checks of the constraint (test/test_correctness.cu:121-181)
+and invariants of the full step, on the oracle and the GPU."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from backends import valid_params
+
+PI = 3.14159265358979323846
+
+
+def weight(d, h):
+    r2 = float(d @ d)
+    return 0.0 if r2 >= h * h else 315.0 / (64.0 * PI * h ** 9) * (h * h - r2) ** 3
+
+
+def gradient(d, h):
+    r = float(np.sqrt(d @ d))
+    if r == 0.0 or r >= h:
+        return np.zeros(3)
+    return -45.0 / (PI * h ** 6) * (h - r) ** 2 / r * d
+
+
+def test_coupled_density_lambda_and_correction_match_fp64(backend):
+    """Non-collinear gradients, a coincident pair and an isolated particle."""
+    p = np.array([[0.49, 0.49, 0.49, 1], [0.51, 0.51, 0.51, 2], [0.25, 0.5, 0.5, 3],
+                  [0.75, 0.5, 0.5, 4], [0.5, 0.25, 0.5, 5], [0.5, 0.75, 0.5, 6],
+                  [0.5, 0.5, 0.25, 7], [0.5, 0.5, 0.75, 8], [0.49, 0.49, 0.49, 9],
+                  [3, 3, 3, 10]], np.float32)
+    n, cap = len(p), 16
+    h, mass, rho0, eps, k, dq = 0.7, 0.4, 2.0, 0.03, 0.002, 0.2
+    g = backend.grid(n, (0, 0, 0), (4, 4, 4), 0.25)
+    sk, si, cs, ce = g.build(p)
+    ps = p[si]
+    nbr, cnt, _ = backend.k.find_neighbors(ps, cs, ce, g.gridSize(), (0, 0, 0), 0.25, n, h, cap, n)
+    lam = backend.k.compute_lambda(ps, nbr, cnt, cap, n, h, mass, rho0, eps, n)
+    dlt = backend.k.compute_delta_position(ps, nbr, cnt, cap, lam, n, h, mass, rho0, k, 4, dq, n)
+    # Back to input order.
+    lam_in = np.empty(n, np.float32)
+    dlt_in = np.empty((n, 4), np.float32)
+    lam_in[si] = lam
+    dlt_in[si] = dlt
+    q = p[:, :3].astype(np.float64)
+    h64, m, r0 = float(np.float32(h)), float(np.float32(mass)), float(np.float32(rho0))
+    ref_lam = np.zeros(n)
+    for i in range(n):
+        rho, den, center = 0.0, float(np.float32(eps)), np.zeros(3)
+        for j in range(n):
+            d = q[i] - q[j]
+            rho += m * weight(d, h64)
+            gj = gradient(d, h64) * m / r0
+            center += gj
+            den += gj @ gj
+        den += center @ center
+        ref_lam[i] = -(rho / r0 - 1.0) / den
+        assert abs(lam_in[i] - ref_lam[i]) <= 2e-6 * max(1.0, abs(ref_lam[i]))
+    wq = weight(np.array([float(np.float32(dq)), 0.0, 0.0]), h64)
+    for i in range(n):
+        corr = np.zeros(3)
+        for j in range(n):
+            if i == j:
+                continue
+            d = q[i] - q[j]
+            sc = -float(np.float32(k)) * (weight(d, h64) / wq) ** 4
+            corr += m / r0 * (ref_lam[i] + ref_lam[j] + sc) * gradient(d, h64)
+        assert np.all(np.abs(dlt_in[i, :3] - corr) <= 2e-5)
+        assert dlt_in[i, 3] == 0.0
+
+
+def test_full_step_is_mirror_symmetric(backend):
+    """Every stage of the step is equivariant under x -> 10 - x, so a
+    configuration symmetric about x = 5 (container, colliders, particles,
+    velocities) stays symmetric with s_corr, XSPH, vorticity and collisions
+    all active.  A wrong sign or index in any stage breaks the symmetry."""
+    from pbf_amd import scenes
+
+    p = scenes.dam_break_params(substeps=1, iterations=4)
+    p.scorrK, p.xsphViscosity, p.vorticityStrength = 1e-3, 1e-3, 1e-3
+    p.collisionFriction, p.collisionRestitution = 0.1, 0.2
+    half = scenes.lattice_block(8, 6, 8, seed=0x5EED0042)[:, :3].astype(np.float64)
+    half[:, 0] -= 1.6  # x in [-1.6, -1.2): mirror x -> -x is exact in FP32
+    half = half.astype(np.float32)
+    m = len(half)
+    mirror = half.copy()
+    mirror[:, 0] = -half[:, 0]
+    pos = np.concatenate([np.concatenate([half, mirror]), np.ones((2 * m, 1), np.float32)], 1)
+    rng = np.random.default_rng(7)
+    vh = (np.tile(np.float32([0.8, 0.0, 0.0]), (m, 1)) +
+          rng.uniform(-0.1, 0.1, (m, 3))).astype(np.float32)
+    vm = vh.copy()
+    vm[:, 0] = -vh[:, 0]
+    vel = np.concatenate([np.concatenate([vh, vm]), np.zeros((2 * m, 1), np.float32)], 1)
+    s = backend.Solver()
+    s.initialize(2 * m, (-2.0, 0.0, 0.0), (2.0, 1.0, 0.4), p)
+    s.setSpheres([((-1.19, 0.12, 0.2), 0.06), ((1.19, 0.12, 0.2), 0.06)])
+    s.setBoxes([((-1.62, 0.2, 0.2), (0.03, 0.06, 0.06)), ((1.62, 0.2, 0.2), (0.03, 0.06, 0.06))])
+    s.setParticles(pos, vel)
+    s.run(3)
+    x, v = s.copyPositionsToHost(), s.copyVelocitiesToHost()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+    a, b = x[:m].astype(np.float64), x[m:].astype(np.float64)
+    va, vb = v[:m].astype(np.float64), v[m:].astype(np.float64)
+    # Summation order differs between mirrored particles, so compare to a
+    # tolerance that FP32 rounding cannot reach but a wrong sign would.
+    assert np.max(np.abs(a[:, 0] + b[:, 0])) <= 1e-5
+    assert np.max(np.abs(a[:, 1:3] - b[:, 1:3])) <= 1e-5
+    assert np.max(np.abs(va[:, 0] + vb[:, 0])) <= 1e-3
+    assert np.max(np.abs(va[:, 1:3] - vb[:, 1:3])) <= 1e-3
+    assert np.max(np.abs(x[:, :3] - pos[:, :3])) > 0.02  # the step did move them

