@@ -6,14 +6,15 @@
 // stable ids map it back to API order on copy-out.  Per substep the device
 // runs (reference stage -> kernel here):
 //   integrate + v_in snapshot + first cell keys      -> k_predict_key
-//   per iteration: grid build, reorder, neighbours   -> radix sort, ranges,
-//                                                       k_gather, k_find_neighbors
-//                  lambda                             -> k_lambda_list
-//                  delta-p, apply, collision solve    -> k_delta_apply_collide
+//   per iteration: grid build, reorder               -> radix sort, ranges, k_gather
+//                  neighbours + lambda                -> sweep_lambda
+//                  delta-p, apply, collision solve    -> sweep_delta_collide
 //   post-solve rebuild (if XSPH or vorticity)         -> as above
-//   velocity update + XSPH                            -> k_update_xsph
-//   vorticity                                         -> k_vorticity_list
-//   confinement + velocity contacts                   -> k_confine_resolve
+//   velocity update + XSPH                            -> sweep_update_xsph
+//   vorticity                                         -> sweep_vorticity
+//   confinement + velocity contacts                   -> sweep_confine_resolve
+// The neighbour passes sweep the 3x3x3 cells directly (k_sweep.hip) with
+// the reference list's neighbour set and order, so no list is stored.
 // Neighbour overflow and collision non-convergence set bits of a device
 // status word; every later kernel that changes the state returns at once,
 // and the host reads the word once when the step (or run) finishes.
@@ -141,14 +142,13 @@ bool finite4(const float4& v) {
 struct PBFSolver::State {
     std::size_t maxParticles = 0;
     std::size_t count = 0;
-    std::size_t stride = 0;
     SimulationParams params{};
     GridCore grid;
     CollisionSystem collision;
 
     DeviceBuffer<float4> x, xa, p, pa, v, va, vin, vina, omega;
-    DeviceBuffer<std::uint32_t> id, ida, nbr, status;
-    DeviceBuffer<int> cnt;
+    DeviceBuffer<std::uint32_t> id, ida, status;
+    DeviceBuffer<std::uint32_t> masks;             // neighbour masks of the last build
     DeviceBuffer<float> lambda;
 
     hipStream_t stream = nullptr;
@@ -202,22 +202,21 @@ struct PBFSolver::State {
         std::swap(id, ida);
     }
 
-    void find_neighbors() {
-        Scope sc(prof, "find_neighbors", stream);
-        launch_check(pbfint::launch_find_neighbors(
-            p.data(), grid.cellStart.data(), grid.cellEnd.data(), grid.dims, grid.minBounds,
-            grid.cellSize, count, params.smoothingRadius, nbr.data(), cnt.data(), kMaxNeighbors,
-            stride, status.data(), PBF_STATUS_NEIGHBOR_OVERFLOW, status.data(), stream));
+    pbfint::SweepGrid sweep_grid() {
+        return pbfint::SweepGrid{grid.cellStart.data(), grid.cellEnd.data(),
+                                 grid.sortedKeys.data(), masks.data(), maxParticles,
+                                 grid.dims, count,
+                                 params.smoothingRadius, kMaxNeighbors};
     }
 
     // One frame: params.substeps substeps (pbf_solver.cu:186-325).
     void enqueue_frame() {
         const SimulationParams& P = params;
-        const std::size_t n = count;
         const float sdt = P.dt / static_cast<float>(P.substeps);
         const float invDt = 1.0f / sdt;
-        const float h = P.smoothingRadius;
         const pbfdev::Colliders col = collision.state().view();
+        const pbfint::SweepGrid sg = sweep_grid();
+        const std::size_t n = count;
         for (int substep = 0; substep < P.substeps; ++substep) {
             {
                 Scope sc(prof, "predict_key", stream);
@@ -228,35 +227,30 @@ struct PBFSolver::State {
             }
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder(it == 0);
-                find_neighbors();
                 {
                     Scope sc(prof, "lambda", stream);
-                    launch_check(pbfint::launch_lambda_guarded(
-                        p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
-                        P.restDensity, P.lambdaEpsilon, lambda.data(), stride, status.data(),
-                        stream));
+                    launch_check(pbfint::sweep_lambda(sg, p.data(), lambda.data(), status.data(),
+                                                      P.particleMass, P.restDensity,
+                                                      P.lambdaEpsilon, stream));
                 }
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
-                    launch_check(pbfint::launch_delta_apply_collide(
-                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, lambda.data(),
-                        n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
-                        stride, col, P.particleRadius, status.data(), stream));
+                    launch_check(pbfint::sweep_delta_collide(
+                        sg, p.data(), lambda.data(), pa.data(), status.data(), P.particleMass,
+                        P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ, col, P.particleRadius,
+                        stream));
                 }
                 std::swap(p, pa);
             }
+            // Rebuild so the velocity passes use the final neighbourhoods
+            // (pbf_solver.cu:260-282).
             const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
-            if (post) {
-                // Rebuild so the velocity passes use the final neighbourhoods
-                // (pbf_solver.cu:260-282).
-                build_and_reorder(false);
-                find_neighbors();
-            }
+            if (post) build_and_reorder(false);
             if (P.xsphViscosity != 0.0f) {
                 Scope sc(prof, "update_xsph", stream);
-                launch_check(pbfint::launch_update_xsph(
-                    x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
-                    n, h, P.xsphViscosity, invDt, stride, status.data(), stream));
+                launch_check(pbfint::sweep_update_xsph(sg, x.data(), p.data(), v.data(),
+                                                       va.data(), status.data(), P.xsphViscosity,
+                                                       invDt, stream));
                 std::swap(v, va);
             } else {
                 Scope sc(prof, "update_velocity", stream);
@@ -268,17 +262,16 @@ struct PBFSolver::State {
             if (P.vorticityStrength != 0.0f) {
                 {
                     Scope sc(prof, "vorticity", stream);
-                    launch_check(pbfint::launch_vorticity(x.data(), v.data(), nbr.data(),
-                                                          cnt.data(), kMaxNeighbors, n, h,
-                                                          omega.data(), stride, stream));
+                    launch_check(pbfint::sweep_vorticity(sg, x.data(), v.data(), omega.data(),
+                                                         status.data(), P.xsphViscosity != 0.0f,
+                                                         stream));
                 }
                 {
                     Scope sc(prof, "confine_resolve", stream);
-                    launch_check(pbfint::launch_confine_resolve(
-                        x.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(), v.data(),
-                        vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
-                        P.particleRadius, P.collisionRestitution, P.collisionFriction,
-                        status.data(), stream));
+                    launch_check(pbfint::sweep_confine_resolve(
+                        sg, x.data(), omega.data(), v.data(), vin.data(), va.data(), status.data(),
+                        sdt, P.vorticityStrength, col, P.particleRadius, P.collisionRestitution,
+                        P.collisionFriction, stream));
                 }
                 std::swap(v, va);
             } else {
@@ -334,7 +327,9 @@ PBFSolver& PBFSolver::operator=(PBFSolver&&) noexcept = default;
 
 void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
                            const SimulationParams& params) {
-    // Validation order and exception types of pbf_solver.cu:25-79.
+    // Validation order and exception types of pbf_solver.cu:25-79.  The
+    // neighbour-list bounds are checked like the reference's although the
+    // sweeps store no list, so the same inputs fail the same way.
     if (maxParticles == 0) throw std::invalid_argument("maxParticles must be greater than zero");
     if (maxParticles > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(kMaxNeighbors))
         throw std::length_error("Neighbor buffer size overflows size_t");
@@ -388,14 +383,12 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
         b->allocate(maxParticles);
     s.id.allocate(maxParticles);
     s.ida.allocate(maxParticles);
-    s.nbr.allocate(stride * static_cast<std::size_t>(kMaxNeighbors));
-    s.cnt.allocate(maxParticles);
     s.lambda.allocate(maxParticles);
     s.status.allocate(1);
+    s.masks.allocate(pbfint::sweep_mask_words(maxParticles));
 
     s.maxParticles = maxParticles;
     s.count = 0;
-    s.stride = stride;
     s.params = params;
 }
 

@@ -17,6 +17,8 @@ pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* 
                                  uint32_t* nbr, int* cnt, int maxN, size_t stride,
                                  uint32_t* flag, uint32_t bit, const uint32_t* guard,
                                  void* stream);
+// Cells along x of one neighbour-sweep tile.
+constexpr uint32_t kSweepTileW = 32;
 pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
 pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream);
 size_t radix_scratch_words(size_t n);
@@ -64,6 +66,44 @@ pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_flo
                          pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
                          uint32_t* ido, size_t n, void* stream);
 pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream);
+
+// Neighbour sweeps (k_sweep.hip): the solver's neighbour passes without
+// lists.  The grid must have cell size == h (a 3x3x3 stencil) and the
+// working set must be in the order of sortedKeys.  Each pass returns at
+// once when *status is non-zero.  The first pass after a grid build
+// (lambda, XSPH, or vorticity without XSPH) tests the radius, raises
+// PBF_STATUS_NEIGHBOR_OVERFLOW when a particle has more than maxNeighbors
+// neighbours and stores its neighbour masks; the passes after it on the
+// same grid and positions (delta-p, confinement, vorticity after XSPH)
+// reuse the masks.
+struct SweepGrid {
+    const int* cellStart;
+    const int* cellEnd;
+    const uint32_t* sortedKeys;
+    uint32_t* masks;            // sweep_mask_words(maskStride) words
+    size_t maskStride;          // >= n
+    pbf_int3 dims;
+    size_t n;
+    float h;
+    int maxNeighbors;
+};
+size_t sweep_mask_words(size_t stride);
+pbf_status sweep_lambda(const SweepGrid& g, const pbf_float4* p, float* lambda, uint32_t* status,
+                        float mass, float rho0, float eps, void* stream);
+pbf_status sweep_delta_collide(const SweepGrid& g, const pbf_float4* p, const float* lambda,
+                               pbf_float4* pout, uint32_t* status, float mass, float rho0,
+                               float sk, int sn, float sdq, const pbfdev::Colliders& col,
+                               float radius, void* stream);
+pbf_status sweep_update_xsph(const SweepGrid& g, const pbf_float4* x, const pbf_float4* p,
+                             const pbf_float4* vold, pbf_float4* vout, uint32_t* status,
+                             float visc, float invDt, void* stream);
+pbf_status sweep_vorticity(const SweepGrid& g, const pbf_float4* x, const pbf_float4* v,
+                           pbf_float4* omega, uint32_t* status, bool reuseMasks, void* stream);
+pbf_status sweep_confine_resolve(const SweepGrid& g, const pbf_float4* x, const pbf_float4* omega,
+                                 const pbf_float4* vcur, const pbf_float4* vsnap,
+                                 pbf_float4* vout, uint32_t* status, float dt, float strength,
+                                 const pbfdev::Colliders& col, float radius, float e, float f,
+                                 void* stream);
 // CollisionSystem::solve as one launch: per-particle fixed point, at most 32
 // passes; sets PBF_STATUS_COLLISION_NONCONVERGENCE in *status otherwise.
 pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const pbfdev::Colliders& col,

@@ -69,10 +69,16 @@ def test_coupled_density_lambda_and_correction_match_fp64(backend):
 
 
 def test_full_step_is_mirror_symmetric(backend):
-    """Every stage of the step is equivariant under x -> 10 - x, so a
-    configuration symmetric about x = 5 (container, colliders, particles,
+    """Every stage of the step is equivariant under x -> -x, so a
+    configuration symmetric about x = 0 (container, colliders, particles,
     velocities) stays symmetric with s_corr, XSPH, vorticity and collisions
-    all active.  A wrong sign or index in any stage breaks the symmetry."""
+    all active.  A wrong sign or index in any stage breaks the symmetry.
+
+    One substep: the scene drives fast particles into colliders with
+    restitution, which amplifies FP32 rounding (mirrored particles sum
+    their neighbours in different orders) by ~100x per substep.  The FP32
+    oracle stays symmetric to 1.2e-7 (positions) and 1.3e-4 (velocities)
+    after one substep and loses it after two; FP64 stays exact."""
     from pbf_amd import scenes
 
     p = scenes.dam_break_params(substeps=1, iterations=4)
@@ -96,7 +102,7 @@ def test_full_step_is_mirror_symmetric(backend):
     s.setSpheres([((-1.19, 0.12, 0.2), 0.06), ((1.19, 0.12, 0.2), 0.06)])
     s.setBoxes([((-1.62, 0.2, 0.2), (0.03, 0.06, 0.06)), ((1.62, 0.2, 0.2), (0.03, 0.06, 0.06))])
     s.setParticles(pos, vel)
-    s.run(3)
+    s.run(1)
     x, v = s.copyPositionsToHost(), s.copyVelocitiesToHost()
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
     a, b = x[:m].astype(np.float64), x[m:].astype(np.float64)
