@@ -73,6 +73,7 @@ void GridCore::sortAndRanges(std::size_t n, hipStream_t stream) {
                                           keyBits, scratch.data(), stream));
     launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
     launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,
+                                            dims, nullptr,
                                             stream));
 }
 

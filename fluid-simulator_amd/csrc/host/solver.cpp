@@ -149,6 +149,7 @@ struct PBFSolver::State {
     DeviceBuffer<float4> x, xa, p, pa, v, va, vin, vina, omega;
     DeviceBuffer<std::uint32_t> id, ida, status;
     DeviceBuffer<std::uint32_t> masks;             // neighbour masks of the last build
+    DeviceBuffer<std::uint32_t> tileEnd;           // sweep tile ends of the last build
     DeviceBuffer<float> lambda;
 
     hipStream_t stream = nullptr;
@@ -186,7 +187,8 @@ struct PBFSolver::State {
             launch_check(pbfint::launch_fill_cells(grid.cellStart.data(), grid.cellEnd.data(),
                                                    grid.numCells, stream));
             launch_check(pbfint::launch_cell_ranges(grid.sortedKeys.data(), grid.cellStart.data(),
-                                                    grid.cellEnd.data(), n, stream));
+                                                    grid.cellEnd.data(), n, grid.dims,
+                                                    tileEnd.data(), stream));
         }
         {
             Scope sc(prof, "gather", stream);
@@ -204,7 +206,8 @@ struct PBFSolver::State {
 
     pbfint::SweepGrid sweep_grid() {
         return pbfint::SweepGrid{grid.cellStart.data(), grid.cellEnd.data(),
-                                 grid.sortedKeys.data(), masks.data(), maxParticles,
+                                 grid.sortedKeys.data(), tileEnd.data(), masks.data(),
+                                 maxParticles,
                                  grid.dims, count,
                                  params.smoothingRadius, kMaxNeighbors};
     }
@@ -386,6 +389,7 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
     s.lambda.allocate(maxParticles);
     s.status.allocate(1);
     s.masks.allocate(pbfint::sweep_mask_words(maxParticles));
+    s.tileEnd.allocate(pbfint::sweep_tile_count(s.grid.dims));
 
     s.maxParticles = maxParticles;
     s.count = 0;

@@ -5,7 +5,7 @@
 // src/pbf/neighbors.cu:3-91 (findNeighbors).
 //
 // Radix sort design (per 8-bit digit pass, reduce-then-scan):
-//   1. k_radix_hist    one 4096-key tile per block; wave64 ballot-match
+//   1. k_radix_hist    one 1024-key tile per block; wave64 ballot-match
 //                      aggregation, one LDS add per distinct digit per wave;
 //                      writes counts[digit * tiles + tile].
 //   2. scan            exclusive scan of the digit-major count table gives
@@ -26,8 +26,8 @@ using namespace pbfdev;
 namespace {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per block
+constexpr int kSortItems = 4;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 1024 keys per block
 constexpr int kRadix = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = kSortThreads * kScanItems;  // 4096 counters per block
@@ -277,14 +277,25 @@ __global__ void __launch_bounds__(kBlock) k_fill_cells(int* __restrict__ cs, int
 }
 
 // buildCellRanges (spatial_grid.cu:123-137): half-open [start, end).
+// With tileEnd not null, also records one past the last particle of every
+// non-empty neighbour-sweep tile (kSweepTileW cells of one row), which the
+// last particle of the tile writes.
 __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restrict__ sk,
                                                         int* __restrict__ cs, int* __restrict__ ce,
-                                                        size_t n) {
+                                                        size_t n, uint32_t gx, uint32_t chunks,
+                                                        uint32_t* __restrict__ tileEnd) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = sk[i];
     if (i == 0 || sk[i - 1] != k) cs[k] = static_cast<int>(i);
-    if (i == n - 1 || sk[i + 1] != k) ce[k] = static_cast<int>(i + 1);
+    const bool lastInCell = i == n - 1 || sk[i + 1] != k;
+    if (!lastInCell) return;
+    ce[k] = static_cast<int>(i + 1);
+    if (tileEnd != nullptr) {
+        const uint32_t t = (k / gx) * chunks + (k % gx) / pbfint::kSweepTileW;
+        if (i == n - 1 || (sk[i + 1] / gx) * chunks + (sk[i + 1] % gx) / pbfint::kSweepTileW != t)
+            tileEnd[t] = static_cast<uint32_t>(i + 1);
+    }
 }
 
 // findNeighbors (src/pbf/neighbors.cu:3-91).  Cells are swept z, y, x over
@@ -365,9 +376,12 @@ pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream) {
     PBF_LAUNCH(k_fill_cells, static_cast<unsigned>(blocks), kBlock, stream, cs, ce, cells);
 }
 
-pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream) {
+pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
+                              uint32_t* tileEnd, void* stream) {
     if (n == 0) return PBF_OK;
-    PBF_LAUNCH(k_cell_ranges, grid_for(n), kBlock, stream, sk, cs, ce, n);
+    const uint32_t gx = static_cast<uint32_t>(dims.x);
+    const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
+    PBF_LAUNCH(k_cell_ranges, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks, tileEnd);
 }
 
 size_t radix_scratch_words(size_t n) {

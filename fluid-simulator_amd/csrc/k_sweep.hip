@@ -28,7 +28,11 @@
 // and positions (the consumers) read them instead of testing again.  A row
 // with more than 64 candidates is not stored and is tested by every pass.
 // A plane whose staged halo exceeds the LDS capacity is read from global
-// memory instead, in the same order.
+// memory instead, in the same order.  A workgroup whose range starts more
+// than kMaxTilePasses tiles (sparse regions: spray, surface) gives each of
+// their particles a lane of its own that reads its 27 cells from global
+// memory, rather than sweeping the tiles one after another; the candidate
+// rows and mask bits are the same in both forms.
 //
 // Requires a 3x3x3 stencil (cell size == smoothing radius), which the
 // solver's grid guarantees (src/pbf/pbf_solver.cu:81-82).
@@ -47,11 +51,13 @@ constexpr int kThreads = 256;
 constexpr int kCap = 1536;             // staged particles per z-plane
 constexpr int kPad = 4;                // test reads run up to 3 records past a row
 constexpr int kRowWords = 2;           // stored mask words per row (64 candidates)
+constexpr int kMaxTilePasses = 4;      // more tiles per workgroup take the lane path
 
 struct Grid {
     const int* cellStart;
     const int* cellEnd;
     const uint32_t* sortedKeys;
+    const uint32_t* tileEnd;
     uint32_t* masks;  // [kSegs * kRowWords][stride]
     size_t n;
     size_t stride;
@@ -70,7 +76,8 @@ struct Meta {
     int cellOff[kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
     int stop;
     int waveTiles[kThreads / 64];
-    int tiles[kThreads];  // tiles starting in this workgroup's particle range
+    int tiles[kThreads];       // tiles starting in this workgroup's particle range
+    int tileFirst[kThreads];   // their first particles
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
@@ -82,6 +89,16 @@ __device__ __forceinline__ int xcd_task(int b, int nb) {
     const int per = nb >> 3;
     const int rem = nb & 7;
     return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+// Workgroup barrier for LDS hand-offs only.  No thread of a sweep reads
+// global memory that another thread of the same launch writes, so the
+// barrier orders LDS alone and leaves global loads in flight across it
+// (__syncthreads would wait for every outstanding global load first).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ __forceinline__ uint32_t tile_of(uint32_t key, uint32_t gx, uint32_t chunks) {
@@ -147,13 +164,13 @@ template <class Pass, bool kConsume, class Src>
 __device__ __forceinline__ void sweep_row(const typename Pass::Args& pa, const Src& src, float4 ip,
                                           float2 ie, int lo, int hi, int selfq, float h2,
                                           typename Pass::Acc& acc, int& count, uint32_t* words,
-                                          size_t stride) {
+                                          size_t stride, uint32_t pre0, uint32_t pre1) {
     const bool kept = hi - lo <= 32 * kRowWords;
     int k = 0;
     for (int c = lo; c < hi; c += 32, ++k) {
         uint32_t mask;
         if (kConsume && kept) {
-            mask = words[static_cast<size_t>(k) * stride];
+            mask = k == 0 ? pre0 : pre1;
         } else {
             const int w = min(32, hi - c);
             mask = 0u;
@@ -246,7 +263,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const typename Pass::A
             m.segCells[s] = incl > 0 ? ncell : 0;
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (tid < 3) {
         int base = 0;
         for (int r = 0; r < 3; ++r) {
@@ -256,7 +273,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const typename Pass::A
         }
         m.planeCount[tid] = base;
     }
-    __syncthreads();
+    lds_barrier();
 
     const int c0 = m.cellOff[4][x0 - xs];
     const int c1 = m.cellOff[4][x1 - xs + 1];
@@ -281,8 +298,30 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const typename Pass::A
             const int pc = m.planeCount[pl];
             if (pc == 0) continue;
             const bool staged = pc <= kCap;
+            // Row ranges of this plane, and for a consumer its stored mask
+            // words, requested before the staging so their latency overlaps.
+            int lo[3], hi[3];
+            uint32_t pre[3][2];
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+                const int s = 3 * pl + rr;
+                const int nc = m.segCells[s];
+                lo[rr] = 0;
+                hi[rr] = 0;
+                pre[rr][0] = 0u;
+                pre[rr][1] = 0u;
+                if (!active || nc == 0) continue;
+                lo[rr] = m.cellOff[s][max(mc - 1, 0)];
+                hi[rr] = m.cellOff[s][min(mc + 1, nc - 1) + 1];
+                const int len = hi[rr] - lo[rr];
+                if (kConsume && len > 0 && len <= 32 * kRowWords) {
+                    const uint32_t* w = g.masks + static_cast<size_t>(s * kRowWords) * g.stride + gi;
+                    pre[rr][0] = w[0];
+                    if (len > 32) pre[rr][1] = w[g.stride];
+                }
+            }
             if (staged) {
-                __syncthreads();  // the previous plane's readers are done
+                lds_barrier();  // the previous plane's readers are done
                 for (int rr = 0; rr < 3; ++rr) {
                     const int s = 3 * pl + rr;
                     if (m.segCells[s] == 0) continue;
@@ -304,26 +343,24 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const typename Pass::A
                         }
                     }
                 }
-                __syncthreads();
+                lds_barrier();
             }
             if (!active) continue;
+#pragma unroll
             for (int rr = 0; rr < 3; ++rr) {
                 const int s = 3 * pl + rr;
-                const int nc = m.segCells[s];
-                if (nc == 0) continue;
-                const int lo = m.cellOff[s][max(mc - 1, 0)];
-                const int hi = m.cellOff[s][min(mc + 1, nc - 1) + 1];
+                if (hi[rr] <= lo[rr]) continue;
                 const int selfq = s == 4 ? r : -1;
                 uint32_t* words = g.masks + static_cast<size_t>(s * kRowWords) * g.stride + gi;
                 if (staged) {
                     const LdsRow<Pass> src{&lds, m.segBase[s]};
-                    sweep_row<Pass, kConsume>(pa, src, ip, ie, lo, hi, selfq, g.h2, acc, count,
-                                              words, g.stride);
+                    sweep_row<Pass, kConsume>(pa, src, ip, ie, lo[rr], hi[rr], selfq, g.h2, acc,
+                                              count, words, g.stride, pre[rr][0], pre[rr][1]);
                 } else {
                     const GlobalRow<Pass> src{&pa, static_cast<size_t>(m.segFirst[s]),
-                                              m.cellOff[s][nc] - 1};
-                    sweep_row<Pass, kConsume>(pa, src, ip, ie, lo, hi, selfq, g.h2, acc, count,
-                                              words, g.stride);
+                                              m.cellOff[s][m.segCells[s]] - 1};
+                    sweep_row<Pass, kConsume>(pa, src, ip, ie, lo[rr], hi[rr], selfq, g.h2, acc,
+                                              count, words, g.stride, pre[rr][0], pre[rr][1]);
                 }
             }
         }
@@ -333,7 +370,55 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const typename Pass::A
             Pass::end(pa, acc, gi, ip, ie);
         }
     }
-    __syncthreads();  // LDS is reused by the next tile
+    lds_barrier();  // LDS is reused by the next tile
+}
+
+// One particle on one lane, candidates read from global memory: the row
+// range of every (dz, dy) is the run of particles from the first non-empty
+// cell of cx-1 .. cx+1 to the last, as in the tile form.
+template <class Pass, bool kConsume>
+__device__ __forceinline__ void sweep_lane(const Grid& g, const typename Pass::Args& pa,
+                                           size_t gi) {
+    const uint32_t key = g.sortedKeys[gi];
+    const uint32_t gx = static_cast<uint32_t>(g.dims.x);
+    const int cx = static_cast<int>(key % gx);
+    const uint32_t rowId = key / gx;
+    const int cy = static_cast<int>(rowId % static_cast<uint32_t>(g.dims.y));
+    const int cz = static_cast<int>(rowId / static_cast<uint32_t>(g.dims.y));
+    float4 ip;
+    float2 ie = make_float2(0.0f, 0.0f);
+    Pass::stage(pa, gi, ip, ie);
+    typename Pass::Acc acc = Pass::begin(pa, ip, ie);
+    int count = 0;
+    const int xa = max(cx - 1, 0);
+    const int xb = min(cx + 1, g.dims.x - 1);
+    for (int s = 0; s < kSegs; ++s) {
+        const int yy = cy + (s % 3) - 1;
+        const int zz = cz + (s / 3) - 1;
+        if (yy < 0 || yy >= g.dims.y || zz < 0 || zz >= g.dims.z) continue;
+        int rowStart = -1, rowEnd = -1;
+        for (int x = xa; x <= xb; ++x) {
+            const uint32_t k = key_of(x, yy, zz, g.dims);
+            const int a = g.cellStart[k];
+            if (a == -1) continue;
+            if (rowStart < 0) rowStart = a;
+            rowEnd = g.cellEnd[k];
+        }
+        if (rowStart < 0) continue;
+        const int len = rowEnd - rowStart;
+        const int selfq = s == 4 ? static_cast<int>(gi - static_cast<size_t>(rowStart)) : -1;
+        const GlobalRow<Pass> src{&pa, static_cast<size_t>(rowStart), len - 1};
+        uint32_t* words = g.masks + static_cast<size_t>(s * kRowWords) * g.stride + gi;
+        uint32_t pre0 = 0u, pre1 = 0u;
+        if (kConsume && len <= 32 * kRowWords) {
+            pre0 = words[0];
+            if (len > 32) pre1 = words[g.stride];
+        }
+        sweep_row<Pass, kConsume>(pa, src, ip, ie, 0, len, selfq, g.h2, acc, count, words,
+                                  g.stride, pre0, pre1);
+    }
+    if (!kConsume && count > g.maxNeighbors) atomicOr(pa.status, PBF_STATUS_NEIGHBOR_OVERFLOW);
+    Pass::end(pa, acc, gi, ip, ie);
 }
 
 // One workgroup per 256 sorted particles: it sweeps every tile whose first
@@ -368,7 +453,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(Grid g, typename Pass::Args 
     // A raised status word stops the step; read once, through LDS, so the
     // whole workgroup takes the decision together.
     if (tid == 0) m.stop = *pa.status != 0u ? 1 : 0;
-    __syncthreads();
+    lds_barrier();
     if (m.stop != 0) return;
     int before = 0, total = 0;
     for (int w = 0; w < kThreads / 64; ++w) {
@@ -380,10 +465,19 @@ __global__ void __launch_bounds__(kThreads) k_sweep(Grid g, typename Pass::Args 
             static_cast<uint32_t>(ballot >> 32),
             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ballot), 0u)));
         m.tiles[before + rank] = static_cast<int>(t);
+        m.tileFirst[before + rank] = static_cast<int>(i);
     }
-    __syncthreads();
-    for (int k = 0; k < total; ++k)
-        sweep_tile<Pass, kConsume>(g, pa, m, lds, m.tiles[k]);
+    lds_barrier();
+    if (total <= kMaxTilePasses) {
+        for (int k = 0; k < total; ++k)
+            sweep_tile<Pass, kConsume>(g, pa, m, lds, m.tiles[k]);
+    } else {
+        // The particles of every tile that starts here, one lane each.
+        const size_t first = static_cast<size_t>(m.tileFirst[0]);
+        const size_t end = g.tileEnd[m.tiles[total - 1]];
+        for (size_t q = first + static_cast<size_t>(tid); q < end; q += kThreads)
+            sweep_lane<Pass, kConsume>(g, pa, q);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -614,6 +708,7 @@ pbf_status launch_sweep(const pbfint::SweepGrid& sg, const typename Pass::Args& 
     g.cellStart = sg.cellStart;
     g.cellEnd = sg.cellEnd;
     g.sortedKeys = sg.sortedKeys;
+    g.tileEnd = sg.tileEnd;
     g.masks = sg.masks;
     g.n = sg.n;
     g.stride = sg.maskStride;
@@ -637,6 +732,11 @@ pbf_status launch_sweep(const pbfint::SweepGrid& sg, const typename Pass::Args& 
 }  // namespace
 
 namespace pbfint {
+
+size_t sweep_tile_count(pbf_int3 dims) {
+    const size_t chunks = (static_cast<size_t>(dims.x) + kTileW - 1) / kTileW;
+    return chunks * static_cast<size_t>(dims.y) * static_cast<size_t>(dims.z);
+}
 
 size_t sweep_mask_words(size_t stride) {
     return static_cast<size_t>(kSegs * kRowWords) * stride;

@@ -20,7 +20,11 @@ pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* 
 // Cells along x of one neighbour-sweep tile.
 constexpr uint32_t kSweepTileW = 32;
 pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
-pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream);
+// Cell ranges of the sorted keys; with tileEnd not null, also the end of
+// every non-empty sweep tile (sweep_tile_count(dims) entries, only those of
+// non-empty tiles written).
+pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
+                              uint32_t* tileEnd, void* stream);
 size_t radix_scratch_words(size_t n);
 int radix_key_bits(size_t numCells);
 pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
@@ -80,6 +84,7 @@ struct SweepGrid {
     const int* cellStart;
     const int* cellEnd;
     const uint32_t* sortedKeys;
+    const uint32_t* tileEnd;    // launch_cell_ranges
     uint32_t* masks;            // sweep_mask_words(maskStride) words
     size_t maskStride;          // >= n
     pbf_int3 dims;
@@ -88,6 +93,7 @@ struct SweepGrid {
     int maxNeighbors;
 };
 size_t sweep_mask_words(size_t stride);
+size_t sweep_tile_count(pbf_int3 dims);
 pbf_status sweep_lambda(const SweepGrid& g, const pbf_float4* p, float* lambda, uint32_t* status,
                         float mass, float rho0, float eps, void* stream);
 pbf_status sweep_delta_collide(const SweepGrid& g, const pbf_float4* p, const float* lambda,

@@ -5,12 +5,14 @@ Bars:
     given identical FP32 input positions;
   * one PBF step: positions within 1e-5 relative per particle; velocities
     within 1e-5 relative in the norm over all particles, and per particle
-    within max(1e-5 |v|, 4 ulp(|x|) / dt).  The per-particle floor is the
+    within max(1e-5 |v|, 6 ulp(|x|) / dt).  The per-particle floor is the
     FP32 storage limit: v = (p* - x) / dt, and a one-ulp rounding split of
     p* in any of the 4 constraint iterations moves v by ulp / dt; an FP32
-    evaluation of the reference's own arithmetic differs from the FP64
-    oracle by up to 2.8 such units (and by 1.16e-5 in the norm, which the
-    FP64 density accumulation of the lambda kernel brings under 1e-5);
+    evaluation of the reference's own arithmetic (the FP32 oracle) differs
+    from the FP64 oracle by up to 2.8 such units on the dam breaks and 4.07
+    inside the tight 3-particle clusters of the spray scene (and by 1.16e-5
+    in the norm, which the FP64 density accumulation of the lambda kernel
+    brings under 1e-5);
   * determinism: bitwise equal states from two solvers;
   * at 1M particles (too slow for the FP64 oracle per step in a test):
     grid bit-exact on the GPU's own p*, stable ids a permutation, state finite
@@ -25,6 +27,10 @@ import pbf_amd
 from pbf_amd import scenes
 
 pytestmark = pytest.mark.gpu
+
+# Per-particle velocity floor in units of ulp(|x|) / dt (see the module
+# docstring): 1.5x the FP32 oracle's worst case.
+ULP_FLOOR = 6.0
 
 
 def _solver_for(scene, cls):
@@ -47,7 +53,7 @@ def _compare_state(pg, vg, po, vo, dt, label):
     dv = np.linalg.norm(vg[:, :3].astype(np.float64) - vo[:, :3], axis=1)
     vn = np.linalg.norm(vo[:, :3].astype(np.float64), axis=1)
     ulp_x = np.spacing(np.abs(po[:, :3]).max(axis=1).astype(np.float32)).astype(np.float64)
-    floor = np.maximum(1e-5 * vn, 4.0 * ulp_x / dt)
+    floor = np.maximum(1e-5 * vn, ULP_FLOOR * ulp_x / dt)
     norm_rel = np.linalg.norm(vg[:, :3] - vo[:, :3]) / max(np.linalg.norm(vo[:, :3]), 1e-30)
     print(f"[{label}] max rel dp {rel_p.max():.3e}  max dv/floor {np.max(dv / floor):.3f}  "
           f"norm rel dv {norm_rel:.3e}  max |dv| {dv.max():.3e}", flush=True)
@@ -116,6 +122,39 @@ def test_one_step_parity_dam_break_config1(gpu):
                    o.copyVelocitiesToHost(), sc.params.dt, "config1 step1")
 
 
+def block_with_spray(clusters=600, seed=11):
+    """A 16^3 dam-break block plus a spray of 3-particle clusters above it.
+
+    In sorted order the spray puts dozens of sparse cell tiles into one
+    256-particle range, which the solver sweeps with one lane per particle;
+    the block's tiles are swept in LDS.  Both forms run in every pass."""
+    sc = scenes.small_block(16, 16, 16)
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform([1.0, 0.9, 0.2], [3.0, 1.4, 0.6], (clusters, 3))
+    offsets = rng.uniform(-0.04, 0.04, (clusters, 3, 3))
+    spray = (centers[:, None, :] + offsets).reshape(-1, 3)
+    vel = rng.uniform(-0.5, 0.5, spray.shape)
+    pos = np.concatenate([sc.positions, np.c_[spray, np.ones(len(spray))].astype(np.float32)])
+    v = np.concatenate([sc.velocities, np.c_[vel, np.zeros(len(vel))].astype(np.float32)])
+    return scenes.Scene("block_with_spray", pos, v, sc.minBounds, sc.maxBounds, sc.params)
+
+
+def test_one_step_parity_block_with_spray(gpu):
+    """One step (every pass, both sweep forms).  The clusters are stiff: the
+    FP32 oracle's own distance from FP64 grows from 4.07 ulp/dt after one
+    step to 6.3 after two and 35 after three, so only one step is held to
+    the strict bars."""
+    from oracle import OracleSolver
+
+    sc = block_with_spray()
+    g = _solver_for(sc, gpu.Solver)
+    o = _solver_for(sc, lambda: OracleSolver("f64"))
+    g.step()
+    o.step()
+    _compare_state(g.copyPositionsToHost(), g.copyVelocitiesToHost(), o.copyPositionsToHost(),
+                   o.copyVelocitiesToHost(), sc.params.dt, "block+spray step1")
+
+
 def _contact_mask(p, sc, band=1e-3):
     """Particles within `band` of a contact: a collider surface inflated by the
     particle radius, or a container wall.  The reference's projection and
@@ -152,9 +191,9 @@ def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True):
     dv = np.linalg.norm(vg[:, :3].astype(np.float64) - vo[:, :3], axis=1)
     vn = np.linalg.norm(vo[:, :3].astype(np.float64), axis=1)
     ulp_x = np.spacing(np.abs(po[:, :3]).max(axis=1).astype(np.float32)).astype(np.float64)
-    strict = (dp <= 1e-5 * pn) & (dv <= np.maximum(1e-5 * vn, 4.0 * ulp_x / dt))
+    strict = (dp <= 1e-5 * pn) & (dv <= np.maximum(1e-5 * vn, ULP_FLOOR * ulp_x / dt))
     contact = _contact_mask(po, sc) | _contact_mask(pg, sc)
-    loose = (dp <= 1e-3 * pn) & (dv <= np.maximum(1e-3 * vn, 4.0 * ulp_x / dt))
+    loose = (dp <= 1e-3 * pn) & (dv <= np.maximum(1e-3 * vn, ULP_FLOOR * ulp_x / dt))
     norm_rel = np.linalg.norm(vg[:, :3] - vo[:, :3]) / max(np.linalg.norm(vo[:, :3]), 1e-30)
     outside = ~strict
     print(f"[{label}] contacts {contact.sum()}  outside strict bar {outside.sum()} "
