@@ -7,18 +7,34 @@ resident in HBM before the timed region.  The throughput unit is the
 particle-step = one particle advanced by one PBF substep; --steps counts
 PBFSolver::step() calls (frames).
 
+Two passes over the same frames from the same seeded state (the step is
+bitwise deterministic): the first is timed end to end for `value`; the
+second records HIP events around every launch on the solver's stream, which
+gives each kernel's average launch duration over exactly the timed frames.
+
 Prints one JSON line:
   value          particle-steps/s over exactly --steps timed steps
-  roofline       dominant kernel's algorithmic bytes / measured launch time
-                 (HIP events on the solver's stream) against 8 TB/s
-  cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on the
-                 host cores, on a bounded sample of the same workload
+  roofline       dominant kernel: algorithmic HBM bytes per launch (bytes
+                 per particle of its phase x particles) / its average launch
+                 duration, against 8 TB/s; `traffic` is the measured HBM
+                 bytes per launch from the committed rocprofv3 counter
+                 summary, when present
+  valu           the same kernel against the unpacked FP32 vector peak
+                 (flop model per neighbour and per candidate test)
+  step_roofline  the whole step's algorithmic bytes per particle-step x
+                 value, against 8 TB/s
+  cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on
+                 the host cores, on a bounded sample of the same workload
+
+`--copy-calibration BYTES` instead copies BYTES of device memory once with
+the library's device-to-device copy: the known-size copy whose HBM counters
+scale the solver kernels' counters.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
+import re
 import statistics
 import sys
 import time
@@ -28,34 +44,63 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "fluid-simulator_amd"))
 
 HBM_PEAK_GBS = 8000.0
-KMEAN = 29  # mean neighbours within h of the workload at rest
+# MI355X FP32 vector FMA peak without packed-FP32 instructions (which this
+# build disables): 256 CUs x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz.
+VALU_PEAK_TFLOPS = 78.6
 
-# Algorithmic HBM bytes per particle for one launch of each step kernel:
-# reads and writes of particle records (16 B), keys/ids/counts (4 B) and
-# neighbour-list slots (4 B each, K of them); neighbour records gathered
-# through the caches are not counted.
-ALG_BYTES = {
-    "predict_key": 32 + 48 + 4,
-    "keys": 16 + 4,
-    "gather": 4 + 4 * 16 + 4 + 4 * 16 + 4,
-    "find_neighbors": 16 + 4 + 4 * KMEAN,
-    "lambda": 16 + 4 + 4 + 4 * KMEAN,
-    "delta_apply_collide": 16 + 4 + 16 + 4 + 4 * KMEAN,
-    "update_xsph": 16 + 16 + 16 + 16 + 4 + 4 * KMEAN,
-    "update_velocity": 16 + 16 + 16 + 16,
-    "vorticity": 16 + 16 + 16 + 4 + 4 * KMEAN,
-    "confine_resolve": 16 + 16 + 16 + 16 + 16 + 4 + 4 * KMEAN,
-    "resolve_velocities": 16 + 16 + 16 + 16,
+# Algorithmic HBM bytes per particle of each step phase: float4 records,
+# 4-byte keys / ids / lambdas, neighbour gathers served on chip and no
+# neighbour list in memory.
+PHASE_BYTES = {
+    "predict_key": 72,          # read x, v; write p*, v_in, key, id
+    "keys": 20,                 # read p*; write key
+    "gather": 104,              # read id + 3 records; write 3 records + id
+    "lambda": 20,               # read p*; write lambda
+    "delta_apply_collide": 36,  # read p*, lambda; write p'
+    "update_xsph": 48,          # read x, p*; write v
+    "update_velocity": 48,
+    "vorticity": 48,            # read p*, v; write omega
+    "confine_resolve": 80,      # read 4 records; write v
+    "resolve_velocities": 64,
 }
+# Flop model per neighbour of each sweep (reference expression trees) and per
+# candidate radius test; KMEAN neighbours and CAND candidates per particle at
+# rest density (8.08 particles per cell, 27 cells).
+FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
+                       "vorticity": 34, "confine_resolve": 32}
+TESTING = {"lambda", "update_xsph"}  # sweeps that run the radius test
+KMEAN, CAND = 29, 218
+TRAFFIC_SUMMARY = REPO / "profiles" / "lambda_sweep_dam_break_1M_traffic.md"
 
 
 def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
     if kernel == "radix_sort":
-        return (16.0 * ((key_bits + 7) // 8) + 4.0 * ((key_bits + 7) // 8)) * n
+        return 16.0 * ((key_bits + 7) // 8) * n
     if kernel == "cell_ranges":
         return 4.0 * n + 8.0 * cells
-    b = ALG_BYTES.get(kernel)
+    b = PHASE_BYTES.get(kernel)
     return None if b is None else float(b) * n
+
+
+def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) -> float:
+    """Algorithmic bytes of one substep of this dataflow per particle: the
+    reference's five grid builds (one per constraint iteration plus the
+    post-solve rebuild), each a key pass, a sort, the ranges and the permute."""
+    builds = iterations + 1
+    build = 16 * ((key_bits + 7) // 8) + 4 + 8 * cells / n + 104
+    return (PHASE_BYTES["predict_key"] + (builds - 1) * PHASE_BYTES["keys"] + builds * build
+            + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
+            + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
+            + PHASE_BYTES["confine_resolve"])
+
+
+def measured_traffic() -> float | None:
+    """HBM bytes per launch of the lambda sweep from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration)."""
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text())
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(scene, seconds_budget: float = 25.0) -> dict:
@@ -88,14 +133,50 @@ def cpu_baseline(scene, seconds_budget: float = 25.0) -> dict:
     }
 
 
+def copy_calibration(nbytes: int) -> int:
+    import ctypes as C
+
+    import pbf_amd
+
+    L = pbf_amd.lib()
+    src, dst = C.c_void_p(), C.c_void_p()
+    pbf_amd._check(L.pbf_device_alloc(C.byref(src), nbytes))
+    pbf_amd._check(L.pbf_device_alloc(C.byref(dst), nbytes))
+    pbf_amd._check(L.pbf_memset(src, 1, nbytes))
+    pbf_amd._check(L.pbf_device_synchronize())
+    pbf_amd._check(L.pbf_memcpy_dtod(dst, src, nbytes))
+    pbf_amd._check(L.pbf_device_synchronize())
+    pbf_amd._check(L.pbf_device_free(src))
+    pbf_amd._check(L.pbf_device_free(dst))
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+    return 0
+
+
+def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool):
+    solver = pbf_amd.PBFSolver()
+    solver.initialize(scene.n, scene.minBounds, scene.maxBounds, scene.params)
+    solver.setParticles(scene.positions, scene.velocities)
+    solver.run(warmup)
+    pbf_amd.synchronize()
+    if profile:
+        solver.setProfiling(True)
+    t0 = time.perf_counter()
+    solver.run(steps)
+    pbf_amd.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = solver.kernelTimes() if profile else {}
+    return elapsed, times
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
-    ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile-pass", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -106,56 +187,60 @@ def main() -> int:
     import pbf_amd
     from pbf_amd import scenes
 
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
+
     scene = scenes.dam_break(a.config)
     n = scene.n
-    solver = pbf_amd.PBFSolver()
-    solver.initialize(n, scene.minBounds, scene.maxBounds, scene.params)
-    solver.setParticles(scene.positions, scene.velocities)
+    substeps = scene.params.substeps
 
     # One timed "step" is one PBFSolver::step() call, i.e. one frame of
     # `substeps` PBF substeps; throughput counts particle-substeps (the
     # reference benchmark's particle-substeps/s, pbf_benchmark.cu:201-211).
-    substeps = scene.params.substeps
-    solver.run(a.warmup)
-    pbf_amd.synchronize()
-    t0 = time.perf_counter()
-    solver.run(a.steps)
-    pbf_amd.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed, _ = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=False)
     value = n * a.steps * substeps / elapsed
-
-    # Per-kernel launch times (HIP events on the solver's stream), after the
-    # timed region so the events do not perturb it.
-    solver.setProfiling(True)
-    for _ in range(a.profile_steps):
-        solver.step()
-    times = solver.kernelTimes()
-    solver.setProfiling(False)
 
     grid = pbf_amd.SpatialGrid()
     grid.initialize(1, scene.minBounds, scene.maxBounds, scene.params.smoothingRadius)
     cells = grid.numCells()
     key_bits = max(0, (cells - 1).bit_length())
+
     kernel_rows = {}
-    for name, (ms, launches) in times.items():
-        avg = ms / max(launches, 1)
-        b = alg_bytes(name, n, key_bits, cells)
-        kernel_rows[name] = {"ms_total": ms, "launches": launches, "avg_ms": avg,
-                             "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
-    dominant = max(kernel_rows, key=lambda k: kernel_rows[k]["ms_total"])
-    drow = kernel_rows[dominant]
-    achieved = drow["gbs"]
-    roofline = {
-        "bound": "hbm",
-        "kernel": dominant,
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-        "traffic": None,
-        "avg_launch_ms": drow["avg_ms"],
-        "alg_bytes_per_launch": alg_bytes(dominant, n, key_bits, cells),
-    }
+    roofline = valu = None
+    if not a.no_profile_pass:
+        _, times = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=True)
+        for name, (ms, launches) in times.items():
+            avg = ms / max(launches, 1)
+            b = alg_bytes(name, n, key_bits, cells)
+            kernel_rows[name] = {"ms_total": ms, "launches": launches, "avg_ms": avg,
+                                 "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
+        dominant = max(kernel_rows, key=lambda k: kernel_rows[k]["ms_total"])
+        drow = kernel_rows[dominant]
+        achieved = drow["gbs"]
+        roofline = {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": measured_traffic() if dominant == "lambda" else None,
+            "avg_launch_ms": drow["avg_ms"],
+            "alg_bytes_per_launch": alg_bytes(dominant, n, key_bits, cells),
+            "launches": drow["launches"],
+        }
+        if dominant in FLOPS_PER_NEIGHBOUR:
+            flops = n * (KMEAN * FLOPS_PER_NEIGHBOUR[dominant]
+                         + (CAND * 8 if dominant in TESTING else 0))
+            tf = flops / (drow["avg_ms"] * 1e-3) / 1e12
+            valu = {"kernel": dominant, "achieved": tf, "peak": VALU_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tf / VALU_PEAK_TFLOPS,
+                    "alg_flops_per_launch": flops}
+
+    bstep = step_bytes_per_particle(scene.params.solverIterations, key_bits, cells, n)
+    step_roof = {"bytes_per_particle_step": bstep,
+                 "achieved": bstep * value / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": bstep * value / 1e9 / HBM_PEAK_GBS}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -183,6 +268,8 @@ def main() -> int:
                    "particles": n, "iterations": scene.params.solverIterations,
                    "substeps": substeps, "parallelism": "single-gpu"},
         "roofline": roofline,
+        "valu": valu,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "kernels": kernel_rows,
     }
