@@ -14,11 +14,11 @@ gives each kernel's average launch duration over exactly the timed frames.
 
 Prints one JSON line:
   value          particle-steps/s over exactly --steps timed steps
-  roofline       dominant kernel: algorithmic HBM bytes per launch (bytes
-                 per particle of its phase x particles) / its average launch
-                 duration, against 8 TB/s; `traffic` is the measured HBM
-                 bytes per launch from the committed rocprofv3 counter
-                 summary, when present
+  roofline       dominant kernel (the neighbour-list build): algorithmic
+                 HBM bytes per launch (bytes per particle of its phase x
+                 particles) / its average launch duration, against 8 TB/s;
+                 `traffic` is the measured HBM bytes per launch from the
+                 committed rocprofv3 counter summary, when present
   valu           the same kernel against the unpacked FP32 vector peak
                  (flop model per neighbour and per candidate test)
   step_roofline  the whole step's algorithmic bytes per particle-step x
@@ -48,29 +48,31 @@ HBM_PEAK_GBS = 8000.0
 # build disables): 256 CUs x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz.
 VALU_PEAK_TFLOPS = 78.6
 
+KMEAN, CAND = 29, 218  # neighbours and candidates per particle at rest
+LIST = 4 + 4 * KMEAN    # a neighbour list: its count and its entries
 # Algorithmic HBM bytes per particle of each step phase: float4 records,
-# 4-byte keys / ids / lambdas, neighbour gathers served on chip and no
-# neighbour list in memory.
+# 4-byte keys / ids / lambdas / list entries, neighbour records gathered on
+# chip (SURVEY.md's phase figures, plus the list the step stores and reads).
 PHASE_BYTES = {
-    "predict_key": 72,          # read x, v; write p*, v_in, key, id
-    "keys": 20,                 # read p*; write key
-    "gather": 104,              # read id + 3 records; write 3 records + id
-    "lambda": 20,               # read p*; write lambda
-    "delta_apply_collide": 36,  # read p*, lambda; write p'
-    "update_xsph": 48,          # read x, p*; write v
+    "predict_key": 72,                 # read x, v; write p*, v_in, key, id
+    "keys": 20,                        # read p*; write key
+    "gather": 104,                     # read id + 3 records; write 3 records + id
+    "build_lists": 20 + LIST,          # read p*, key; write the list
+    "lambda": 20 + LIST,               # read p*, the list; write lambda
+    "delta_apply_collide": 36 + LIST,  # read p*, lambda, the list; write p'
+    "update_xsph": 48 + LIST,          # read x, p*, the list; write v
     "update_velocity": 48,
-    "vorticity": 48,            # read p*, v; write omega
-    "confine_resolve": 80,      # read 4 records; write v
+    "vorticity": 48 + LIST,            # read p*, v, the list; write omega
+    "confine_resolve": 80 + LIST,      # read 4 records, the list; write v
     "resolve_velocities": 64,
 }
-# Flop model per neighbour of each sweep (reference expression trees) and per
-# candidate radius test; KMEAN neighbours and CAND candidates per particle at
-# rest density (8.08 particles per cell, 27 cells).
+# Flop model: per neighbour of each pass (reference expression
+# trees) and 8 per candidate radius test.
 FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
-                       "vorticity": 34, "confine_resolve": 32}
-TESTING = {"lambda", "update_xsph"}  # sweeps that run the radius test
-KMEAN, CAND = 29, 218
-TRAFFIC_SUMMARY = REPO / "profiles" / "lambda_sweep_dam_break_1M_traffic.md"
+                       "vorticity": 34, "confine_resolve": 32, "build_lists": 0}
+TESTING = {"build_lists"}  # passes that run the radius test
+TRAFFIC_SUMMARY = REPO / "profiles" / "build_lists_dam_break_1M_traffic.md"
+TRAFFIC_KERNEL = "build_lists"
 
 
 def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
@@ -85,9 +87,11 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
 def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) -> float:
     """Algorithmic bytes of one substep of this dataflow per particle: the
     reference's five grid builds (one per constraint iteration plus the
-    post-solve rebuild), each a key pass, a sort, the ranges and the permute."""
+    post-solve rebuild), each a key pass, a sort, the ranges, the permute
+    and the neighbour lists."""
     builds = iterations + 1
-    build = 16 * ((key_bits + 7) // 8) + 4 + 8 * cells / n + 104
+    build = (16 * ((key_bits + 7) // 8) + 4 + 8 * cells / n + PHASE_BYTES["gather"]
+             + PHASE_BYTES["build_lists"])
     return (PHASE_BYTES["predict_key"] + (builds - 1) * PHASE_BYTES["keys"] + builds * build
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
             + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
@@ -95,7 +99,7 @@ def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) 
 
 
 def measured_traffic() -> float | None:
-    """HBM bytes per launch of the lambda sweep from the committed counter
+    """HBM bytes per launch of the list build from the committed counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration)."""
     if not TRAFFIC_SUMMARY.exists():
         return None
@@ -224,7 +228,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": measured_traffic() if dominant == "lambda" else None,
+            "traffic": measured_traffic() if dominant == TRAFFIC_KERNEL else None,
             "avg_launch_ms": drow["avg_ms"],
             "alg_bytes_per_launch": alg_bytes(dominant, n, key_bits, cells),
             "launches": drow["launches"],

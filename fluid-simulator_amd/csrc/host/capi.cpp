@@ -273,4 +273,12 @@ pbf_status pbf_solver_kernel_times(pbf_solver* s, char* names, size_t namesBytes
     });
 }
 
+pbf_status pbf_solver_neighbor_lists(pbf_solver* s, pbf_float4* positions, int* counts,
+                                     uint32_t* neighbors, size_t n, int maxNeighbors) {
+    PBF_REQUIRE(s);
+    return guarded([&] {
+        s->solver.copyNeighborLists(positions, counts, neighbors, n, maxNeighbors);
+    });
+}
+
 }  // extern "C"

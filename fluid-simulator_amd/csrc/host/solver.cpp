@@ -7,14 +7,16 @@
 // runs (reference stage -> kernel here):
 //   integrate + v_in snapshot + first cell keys      -> k_predict_key
 //   per iteration: grid build, reorder               -> radix sort, ranges, k_gather
-//                  neighbours + lambda                -> sweep_lambda
-//                  delta-p, apply, collision solve    -> sweep_delta_collide
+//                  neighbour lists                    -> k_build_lists (LDS cell sweep)
+//                  lambda                             -> k_lambda_list
+//                  delta-p, apply, collision solve    -> k_delta_apply_collide
 //   post-solve rebuild (if XSPH or vorticity)         -> as above
-//   velocity update + XSPH                            -> sweep_update_xsph
-//   vorticity                                         -> sweep_vorticity
-//   confinement + velocity contacts                   -> sweep_confine_resolve
-// The neighbour passes sweep the 3x3x3 cells directly (k_sweep.hip) with
-// the reference list's neighbour set and order, so no list is stored.
+//   velocity update + XSPH                            -> k_update_xsph
+//   vorticity                                         -> k_vorticity_list
+//   confinement + velocity contacts                   -> k_confine_resolve
+// The lists are the reference's (slot-major, same neighbours in the same
+// order); the list kernels run their fast form (hardware rsq in the spiky
+// gradient, workgroups contiguous per XCD).
 // Neighbour overflow and collision non-convergence set bits of a device
 // status word; every later kernel that changes the state returns at once,
 // and the host reads the word once when the step (or run) finishes.
@@ -148,8 +150,10 @@ struct PBFSolver::State {
 
     DeviceBuffer<float4> x, xa, p, pa, v, va, vin, vina, omega;
     DeviceBuffer<std::uint32_t> id, ida, status;
-    DeviceBuffer<std::uint32_t> masks;             // neighbour masks of the last build
-    DeviceBuffer<std::uint32_t> tileEnd;           // sweep tile ends of the last build
+    DeviceBuffer<std::uint32_t> nbr;      // slot-major neighbour lists of the last build
+    DeviceBuffer<int> cnt;
+    std::size_t stride = 0;
+    DeviceBuffer<std::uint32_t> tileEnd;  // sweep tile ends of the last build
     DeviceBuffer<float> lambda;
 
     hipStream_t stream = nullptr;
@@ -206,20 +210,25 @@ struct PBFSolver::State {
 
     pbfint::SweepGrid sweep_grid() {
         return pbfint::SweepGrid{grid.cellStart.data(), grid.cellEnd.data(),
-                                 grid.sortedKeys.data(), tileEnd.data(), masks.data(),
-                                 maxParticles,
-                                 grid.dims, count,
+                                 grid.sortedKeys.data(), tileEnd.data(), grid.dims, count,
                                  params.smoothingRadius, kMaxNeighbors};
+    }
+
+    // Neighbour lists by cell sweep, then the flat list kernels.
+    void build_lists() {
+        Scope sc(prof, "build_lists", stream);
+        launch_check(pbfint::sweep_build_lists(sweep_grid(), p.data(), nbr.data(), cnt.data(),
+                                               stride, status.data(), stream));
     }
 
     // One frame: params.substeps substeps (pbf_solver.cu:186-325).
     void enqueue_frame() {
         const SimulationParams& P = params;
+        const std::size_t n = count;
         const float sdt = P.dt / static_cast<float>(P.substeps);
         const float invDt = 1.0f / sdt;
+        const float h = P.smoothingRadius;
         const pbfdev::Colliders col = collision.state().view();
-        const pbfint::SweepGrid sg = sweep_grid();
-        const std::size_t n = count;
         for (int substep = 0; substep < P.substeps; ++substep) {
             {
                 Scope sc(prof, "predict_key", stream);
@@ -230,51 +239,54 @@ struct PBFSolver::State {
             }
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder(it == 0);
+                build_lists();
                 {
                     Scope sc(prof, "lambda", stream);
-                    launch_check(pbfint::sweep_lambda(sg, p.data(), lambda.data(), status.data(),
-                                                      P.particleMass, P.restDensity,
-                                                      P.lambdaEpsilon, stream));
+                    launch_check(pbfint::launch_lambda_guarded(
+                        p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
+                        P.restDensity, P.lambdaEpsilon, lambda.data(), stride, status.data(),
+                        stream));
                 }
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
-                    launch_check(pbfint::sweep_delta_collide(
-                        sg, p.data(), lambda.data(), pa.data(), status.data(), P.particleMass,
-                        P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ, col, P.particleRadius,
-                        stream));
+                    launch_check(pbfint::launch_delta_apply_collide(
+                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, lambda.data(),
+                        n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
+                        stride, col, P.particleRadius, status.data(), stream));
                 }
                 std::swap(p, pa);
             }
-            // Rebuild so the velocity passes use the final neighbourhoods
-            // (pbf_solver.cu:260-282).
             const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
-            if (post) build_and_reorder(false);
+            if (post) {
+                build_and_reorder(false);
+                build_lists();
+            }
             if (P.xsphViscosity != 0.0f) {
                 Scope sc(prof, "update_xsph", stream);
-                launch_check(pbfint::sweep_update_xsph(sg, x.data(), p.data(), v.data(),
-                                                       va.data(), status.data(), P.xsphViscosity,
-                                                       invDt, stream));
+                launch_check(pbfint::launch_update_xsph(
+                    x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
+                    n, h, P.xsphViscosity, invDt, stride, status.data(), stream));
                 std::swap(v, va);
             } else {
                 Scope sc(prof, "update_velocity", stream);
                 launch_check(pbfint::launch_update_velocity_only(x.data(), p.data(), v.data(), n,
                                                                  invDt, status.data(), stream));
             }
-            // positions <- predicted positions (pbf_solver.cu:210 of the kernel).
             std::swap(x, p);
             if (P.vorticityStrength != 0.0f) {
                 {
                     Scope sc(prof, "vorticity", stream);
-                    launch_check(pbfint::sweep_vorticity(sg, x.data(), v.data(), omega.data(),
-                                                         status.data(), P.xsphViscosity != 0.0f,
-                                                         stream));
+                    launch_check(pbfint::launch_vorticity(x.data(), v.data(), nbr.data(),
+                                                          cnt.data(), kMaxNeighbors, n, h,
+                                                          omega.data(), stride, stream));
                 }
                 {
                     Scope sc(prof, "confine_resolve", stream);
-                    launch_check(pbfint::sweep_confine_resolve(
-                        sg, x.data(), omega.data(), v.data(), vin.data(), va.data(), status.data(),
-                        sdt, P.vorticityStrength, col, P.particleRadius, P.collisionRestitution,
-                        P.collisionFriction, stream));
+                    launch_check(pbfint::launch_confine_resolve(
+                        x.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(), v.data(),
+                        vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
+                        P.particleRadius, P.collisionRestitution, P.collisionFriction,
+                        status.data(), stream));
                 }
                 std::swap(v, va);
             } else {
@@ -388,7 +400,9 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
     s.ida.allocate(maxParticles);
     s.lambda.allocate(maxParticles);
     s.status.allocate(1);
-    s.masks.allocate(pbfint::sweep_mask_words(maxParticles));
+    s.nbr.allocate(stride * static_cast<std::size_t>(kMaxNeighbors));
+    s.cnt.allocate(maxParticles);
+    s.stride = stride;
     s.tileEnd.allocate(pbfint::sweep_tile_count(s.grid.dims));
 
     s.maxParticles = maxParticles;
@@ -514,5 +528,26 @@ void PBFSolver::setProfiling(bool enabled) {
 }
 
 std::vector<PBFSolver::KernelTime> PBFSolver::kernelTimes() const { return state_->prof.times(); }
+
+void PBFSolver::copyNeighborLists(float4* positions, int* counts, std::uint32_t* neighbors,
+                                  std::size_t n, int maxNeighbors) const {
+    State& s = *state_;
+    if (s.maxParticles == 0)
+        throw std::logic_error("PBFSolver must be initialized before reading neighbour lists");
+    if (n != s.count)
+        throw std::out_of_range("Requested neighbour list count does not match active particles");
+    if (maxNeighbors < 0 || maxNeighbors > kMaxNeighbors)
+        throw std::invalid_argument("maxNeighbors must lie in [0, 256]");
+    if (n == 0) return;
+    if (positions == nullptr || counts == nullptr || (maxNeighbors > 0 && neighbors == nullptr))
+        throw std::invalid_argument("Destination cannot be null");
+    hip_check(hipStreamSynchronize(s.stream));
+    hip_check(hipMemcpy(positions, s.x.data(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    hip_check(hipMemcpy(counts, s.cnt.data(), n * sizeof(int), hipMemcpyDeviceToHost));
+    if (maxNeighbors > 0)
+        hip_check(hipMemcpy2D(neighbors, n * sizeof(std::uint32_t), s.nbr.data(),
+                              s.stride * sizeof(std::uint32_t), n * sizeof(std::uint32_t),
+                              static_cast<std::size_t>(maxNeighbors), hipMemcpyDeviceToHost));
+}
 
 }  // namespace pbf

@@ -12,6 +12,21 @@ using namespace pbfdev;
 
 namespace {
 
+// The step's list kernels run the kFast form: the hardware reciprocal square
+// root in the spiky gradient (spiky_grad_rsq) and workgroups laid out
+// contiguously per XCD.  The C ABI launchers run the reference-shaped form
+// (correctly rounded sqrt and division, plain block order).
+template <bool kFast>
+__device__ __forceinline__ float3 grad(float dx, float dy, float dz, float d2, const Sph& s) {
+    return kFast ? spiky_grad_rsq(dx, dy, dz, d2, s) : spiky_grad(dx, dy, dz, d2, s);
+}
+
+template <bool kFast>
+__device__ __forceinline__ size_t thread_index() {
+    const unsigned b = kFast ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    return static_cast<size_t>(b) * blockDim.x + threadIdx.x;
+}
+
 // ---------------------------------------------------------------------------
 // Reference-shaped kernels
 // ---------------------------------------------------------------------------
@@ -38,12 +53,13 @@ __global__ void __launch_bounds__(kBlock) k_integrate(const pbf_float4* __restri
 }
 
 // computeLambda (src/pbf/pbf_solver_kernels.cu:26-105).
+template <bool kFast>
 __global__ void __launch_bounds__(kBlock)
     k_lambda_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
                   const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
                   float eps, float* __restrict__ lambda, size_t stride,
                   const uint32_t* __restrict__ status) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = thread_index<kFast>();
     if (i >= n) return;
     if (status != nullptr && *status != 0u) return;
     const float4 pi = ld4(p, i);
@@ -65,7 +81,7 @@ __global__ void __launch_bounds__(kBlock)
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float d2 = dx * dx + dy * dy + dz * dz;
         rho += static_cast<double>(mass * poly6_d2(d2, s));
-        const float3 gw = spiky_grad(dx, dy, dz, d2, s);
+        const float3 gw = grad<kFast>(dx, dy, dz, d2, s);
         const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
         sg += static_cast<double>(gjx * gjx + gjy * gjy + gjz * gjz);
         gix -= static_cast<double>(gjx);
@@ -174,12 +190,15 @@ __global__ void __launch_bounds__(kBlock)
     st4(vout, i, make_float4(vi.x + visc * cx, vi.y + visc * cy, vi.z + visc * cz, vi.w));
 }
 
-// computeVorticity (src/pbf/pbf_solver_kernels.cu:268-315).
+// computeVorticity (src/pbf/pbf_solver_kernels.cu:268-315).  The kFast form
+// also stores |omega| in .w (the reference writes 0 there), which the
+// step's confinement kernel reads instead of recomputing it per neighbour.
+template <bool kFast>
 __global__ void __launch_bounds__(kBlock)
     k_vorticity_list(const pbf_float4* __restrict__ p, const pbf_float4* __restrict__ v,
                      const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                      size_t n, float h, pbf_float4* __restrict__ omega, size_t stride) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = thread_index<kFast>();
     if (i >= n) return;
     const int c = cnt[i];
     if (!valid_count(c, maxN)) {
@@ -195,18 +214,21 @@ __global__ void __launch_bounds__(kBlock)
         const float4 pj = ld4(p, j);
         const float4 vj = ld4(v, j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float3 g = spiky_grad(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+        const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         ox += g.y * uz - g.z * uy;
         oy += g.z * ux - g.x * uz;
         oz += g.x * uy - g.y * ux;
     }
-    st4(omega, i, make_float4(ox, oy, oz, 0.0f));
+    const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
+    st4(omega, i, make_float4(ox, oy, oz, wl));
 }
 
 // applyVorticityConfinement (src/pbf/pbf_solver_kernels.cu:317-385) as a
 // device function, shared by the API kernel and the fused confinement +
-// contact kernel.  Returns false for an invalid neighbour count.
+// contact kernel.  The kFast form takes |omega| from .w (written by the
+// kFast vorticity kernel).  Returns false for an invalid neighbour count.
+template <bool kFast>
 __device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__ p,
                                         const uint32_t* __restrict__ nbr, int c,
                                         const pbf_float4* __restrict__ omega, float4 vi,
@@ -214,15 +236,20 @@ __device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__
                                         float4& out) {
     const float4 pi = ld4(p, i);
     const float4 wi = ld4(omega, i);
-    const float wil = __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
+    const float wil = kFast ? wi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     for (int q = 0; q < c; ++q) {
         const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
         const float4 pj = ld4(p, j);
-        const float4 wj = ld4(omega, j);
-        const float wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
+        float wjl;
+        if (kFast) {
+            wjl = omega[j].w;
+        } else {
+            const float4 wj = ld4(omega, j);
+            wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
+        }
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float3 g = spiky_grad(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+        const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
         const float md = wjl - wil;
         ex += md * g.x;
         ey += md * g.y;
@@ -257,7 +284,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     const Sph s = sph_consts(h);
     float4 out;
-    confine(i, p, nbr, c, omega, vi, s, dt, strength, stride, out);
+    confine<false>(i, p, nbr, c, omega, vi, s, dt, strength, stride, out);
     st4(vout, i, out);
 }
 
@@ -434,7 +461,7 @@ __global__ void __launch_bounds__(kBlock)
                           int maxN, const float* __restrict__ lambda, size_t n, float h,
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
                           Colliders col, float radius, uint32_t* __restrict__ status) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
     float4 pi = ld4(pin, i);
@@ -452,7 +479,7 @@ __global__ void __launch_bounds__(kBlock)
             const float4 pj = ld4(pin, j);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            const float3 gw = spiky_grad(dx, dy, dz, d2, s);
+            const float3 gw = grad<true>(dx, dy, dz, d2, s);
             const float ls = li + lambda[j] + scorr_value(d2, s.h2, sc);
             ax += ls * gw.x;
             ay += ls * gw.y;
@@ -482,7 +509,7 @@ __global__ void __launch_bounds__(kBlock)
                   const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                   size_t n, float h, float visc, float inv_dt, size_t stride,
                   const uint32_t* __restrict__ status) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
     const float4 xi = ld4(x, i);
@@ -539,7 +566,7 @@ __global__ void __launch_bounds__(kBlock)
                       pbf_float4* __restrict__ vout, size_t n, float h, float dt, float strength,
                       size_t stride, Colliders col, float radius, float e, float f,
                       const uint32_t* __restrict__ status) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
     const float4 vi = ld4(vcur, i);
@@ -549,7 +576,7 @@ __global__ void __launch_bounds__(kBlock)
         out = make_float4(qnan(), qnan(), qnan(), vi.w);
     } else {
         const Sph s = sph_consts(h);
-        confine(i, x, nbr, c, omega, vi, s, dt, strength, stride, out);
+        confine<true>(i, x, nbr, c, omega, vi, s, dt, strength, stride, out);
     }
     resolve_velocity(ld4(x, i), ld4(vsnap, i), out, col, radius, e, f);
     st4(vout, i, out);
@@ -604,8 +631,8 @@ pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const 
                                 int maxN, size_t n, float h, float mass, float rho0, float eps,
                                 float* lambda, size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
-    PBF_LAUNCH(k_lambda_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass, rho0,
-               eps, lambda, stride, static_cast<const uint32_t*>(nullptr));
+    PBF_LAUNCH(k_lambda_list<false>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
+               rho0, eps, lambda, stride, static_cast<const uint32_t*>(nullptr));
 }
 
 pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
@@ -640,8 +667,8 @@ pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, con
                                    const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
                                    size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
-    PBF_LAUNCH(k_vorticity_list, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h, omega,
-               stride);
+    PBF_LAUNCH(k_vorticity_list<false>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
+               omega, stride);
 }
 
 pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
@@ -761,8 +788,8 @@ pbf_status launch_lambda_guarded(const pbf_float4* p, const uint32_t* nbr, const
                                  int maxN, size_t n, float h, float mass, float rho0, float eps,
                                  float* lambda, size_t stride, const uint32_t* status,
                                  void* stream) {
-    PBF_LAUNCH(k_lambda_list, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass, rho0,
-               eps, lambda, stride, status);
+    PBF_LAUNCH(k_lambda_list<true>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
+               rho0, eps, lambda, stride, status);
 }
 
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
@@ -791,8 +818,8 @@ pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p,
 pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
                             size_t stride, void* stream) {
-    PBF_LAUNCH(k_vorticity_list, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h, omega,
-               stride);
+    PBF_LAUNCH(k_vorticity_list<true>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
+               omega, stride);
 }
 
 pbf_status launch_confine_resolve(const pbf_float4* x, const uint32_t* nbr, const int* cnt,

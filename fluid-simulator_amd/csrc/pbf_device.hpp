@@ -441,6 +441,18 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
 // Launch geometry: 256 threads (4 wave64) per block, the reference's kBlockSize.
 constexpr int kBlock = 256;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8).  xcd_block(b, nb) gives every XCD one contiguous run of block
+// indices, so a kernel whose neighbouring blocks read overlapping data (the
+// cell-sorted working set) keeps that data in one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned xcd = b & 7u;
+    const unsigned k = b >> 3;
+    const unsigned per = nb >> 3;
+    const unsigned rem = nb & 7u;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
 inline unsigned grid_for(size_t n, int block = kBlock) {
     return static_cast<unsigned>((n + static_cast<size_t>(block) - 1) / static_cast<size_t>(block));
 }

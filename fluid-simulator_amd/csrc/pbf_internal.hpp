@@ -71,45 +71,26 @@ pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_flo
                          uint32_t* ido, size_t n, void* stream);
 pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream);
 
-// Neighbour sweeps (k_sweep.hip): the solver's neighbour passes without
-// lists.  The grid must have cell size == h (a 3x3x3 stencil) and the
-// working set must be in the order of sortedKeys.  Each pass returns at
-// once when *status is non-zero.  The first pass after a grid build
-// (lambda, XSPH, or vorticity without XSPH) tests the radius, raises
-// PBF_STATUS_NEIGHBOR_OVERFLOW when a particle has more than maxNeighbors
-// neighbours and stores its neighbour masks; the passes after it on the
-// same grid and positions (delta-p, confinement, vorticity after XSPH)
-// reuse the masks.
+// Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
+// have cell size == h (a 3x3x3 stencil) and the working set must be in the
+// order of sortedKeys; tileEnd comes from launch_cell_ranges.
 struct SweepGrid {
     const int* cellStart;
     const int* cellEnd;
     const uint32_t* sortedKeys;
-    const uint32_t* tileEnd;    // launch_cell_ranges
-    uint32_t* masks;            // sweep_mask_words(maskStride) words
-    size_t maskStride;          // >= n
+    const uint32_t* tileEnd;
     pbf_int3 dims;
     size_t n;
     float h;
     int maxNeighbors;
 };
-size_t sweep_mask_words(size_t stride);
+// Entries of the tileEnd array for a grid of these dims.
 size_t sweep_tile_count(pbf_int3 dims);
-pbf_status sweep_lambda(const SweepGrid& g, const pbf_float4* p, float* lambda, uint32_t* status,
-                        float mass, float rho0, float eps, void* stream);
-pbf_status sweep_delta_collide(const SweepGrid& g, const pbf_float4* p, const float* lambda,
-                               pbf_float4* pout, uint32_t* status, float mass, float rho0,
-                               float sk, int sn, float sdq, const pbfdev::Colliders& col,
-                               float radius, void* stream);
-pbf_status sweep_update_xsph(const SweepGrid& g, const pbf_float4* x, const pbf_float4* p,
-                             const pbf_float4* vold, pbf_float4* vout, uint32_t* status,
-                             float visc, float invDt, void* stream);
-pbf_status sweep_vorticity(const SweepGrid& g, const pbf_float4* x, const pbf_float4* v,
-                           pbf_float4* omega, uint32_t* status, bool reuseMasks, void* stream);
-pbf_status sweep_confine_resolve(const SweepGrid& g, const pbf_float4* x, const pbf_float4* omega,
-                                 const pbf_float4* vcur, const pbf_float4* vsnap,
-                                 pbf_float4* vout, uint32_t* status, float dt, float strength,
-                                 const pbfdev::Colliders& col, float radius, float e, float f,
-                                 void* stream);
+// The list launch_find_neighbors builds (nbr[q * stride + i], at most
+// maxNeighbors stored, counts whole), bit for bit; returns at once when
+// *status is non-zero and raises PBF_STATUS_NEIGHBOR_OVERFLOW.
+pbf_status sweep_build_lists(const SweepGrid& g, const pbf_float4* p, uint32_t* nbr, int* cnt,
+                             size_t stride, uint32_t* status, void* stream);
 // CollisionSystem::solve as one launch: per-particle fixed point, at most 32
 // passes; sets PBF_STATUS_COLLISION_NONCONVERGENCE in *status otherwise.
 pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const pbfdev::Colliders& col,

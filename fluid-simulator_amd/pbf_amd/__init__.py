@@ -244,6 +244,7 @@ _SIGNATURES = {
     "pbf_solver_set_profiling": (_st, [_vp, _i]),
     "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
+    "pbf_solver_neighbor_lists": (_st, [_vp, _vp, _vp, _vp, _sz, _i]),
 }
 
 _LIB = None
@@ -597,6 +598,17 @@ class PBFSolver:
                                              C.byref(count)))
         keys = names.value.decode().split("\n")[:count.value]
         return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
+
+    def neighborLists(self, maxNeighbors: int = 256):
+        """(positions, counts, neighbours) of the last grid build: working-set
+        order, neighbours slot-major as (maxNeighbors, n)."""
+        n = self.particleCount()
+        pos = np.zeros((n, 4), np.float32)
+        cnt = np.zeros(n, np.int32)
+        nbr = np.zeros((maxNeighbors, n), np.uint32)
+        _check(lib().pbf_solver_neighbor_lists(self._h, pos.ctypes.data, cnt.ctypes.data,
+                                               nbr.ctypes.data, n, int(maxNeighbors)))
+        return pos, cnt, nbr
 
 
 from . import kernels  # noqa: E402  (device kernel wrappers need the classes above)

@@ -62,6 +62,12 @@ public:
     };
     std::vector<KernelTime> kernelTimes() const;
 
+    // The last grid build's neighbour lists, slot-major with stride
+    // particleCount, and the working-set positions they were built on
+    // (see pbf_solver_neighbor_lists).
+    void copyNeighborLists(float4* positions, int* counts, std::uint32_t* neighbors,
+                           std::size_t particleCount, int maxNeighbors) const;
+
     struct State;  // library-internal
 
 private:

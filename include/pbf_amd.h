@@ -363,6 +363,18 @@ pbf_status pbf_solver_kernel_times(pbf_solver* solver, char* names, size_t names
                                    double* milliseconds, uint64_t* launches, size_t capacity,
                                    size_t* kernelCount);
 
+/* Introspection, no counterpart in the reference: the neighbour lists of the
+ * solver's last grid build, in the reference's slot-major layout
+ * (h_neighbors[q * particleCount + i], q < min(h_counts[i], maxNeighbors)),
+ * together with the working-set positions they were built on (working-set
+ * order, not API order).  After a frame with XSPH or vorticity enabled the
+ * last build is the post-solve rebuild, whose positions are the current
+ * ones.  particleCount must equal the active count; maxNeighbors caps the
+ * slots copied (h_neighbors holds maxNeighbors * particleCount entries). */
+pbf_status pbf_solver_neighbor_lists(pbf_solver* solver, pbf_float4* h_positions, int* h_counts,
+                                     uint32_t* h_neighbors, size_t particleCount,
+                                     int maxNeighbors);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,6 +155,27 @@ def test_one_step_parity_block_with_spray(gpu):
                    o.copyVelocitiesToHost(), sc.params.dt, "block+spray step1")
 
 
+@pytest.mark.parametrize("scene_name", ["dam_break_1", "block_with_spray"])
+def test_step_neighbor_lists_equal_find_neighbors(gpu, oracle64, scene_name):
+    """The step builds its lists by LDS cell sweep (tile form and, for the
+    spray, the per-lane form); after two frames the last build's lists equal
+    findNeighbors' on the same positions, entry by entry."""
+    sc = scenes.dam_break(1) if scene_name == "dam_break_1" else block_with_spray()
+    s = _solver_for(sc, gpu.Solver)
+    s.run(2)
+    pos, cnt, nbr = s.neighborLists()
+    n, h = sc.n, sc.params.smoothingRadius
+    osk, osi, ocs, oce, dims, cells = oracle64.grid_build(pos, sc.minBounds, sc.maxBounds, h)
+    assert np.array_equal(osi, np.arange(n)), "the working set is in cell order"
+    on, oc, of = oracle64.find_neighbors(pos, ocs, oce, dims, sc.minBounds, h, n, h, 256, n)
+    assert of == 0
+    assert np.array_equal(cnt, oc)
+    on = on.reshape(256, n)
+    mask = np.arange(256)[:, None] < oc[None, :]
+    assert np.array_equal(nbr[mask], on[mask])
+    print(f"[{scene_name}] {n} particles, {int(oc.sum())} neighbour entries equal", flush=True)
+
+
 def _contact_mask(p, sc, band=1e-3):
     """Particles within `band` of a contact: a collider surface inflated by the
     particle radius, or a container wall.  The reference's projection and
