@@ -4,13 +4,14 @@
 // Reference: src/pbf/spatial_grid.cu:100-226 (keys, CUB SortPairs, ranges),
 // src/pbf/neighbors.cu:3-91 (findNeighbors).
 //
-// Radix sort design (per 8-bit digit pass, reduce-then-scan):
+// Radix sort design (per 8-bit digit pass, three launches):
 //   1. k_radix_hist    one 1024-key tile per block; wave64 ballot-match
 //                      aggregation, one LDS add per distinct digit per wave;
 //                      writes counts[digit * tiles + tile].
-//   2. scan            exclusive scan of the digit-major count table gives
-//                      every (digit, tile) its global output offset.
-//   3. k_radix_scatter ranks keys stably inside the tile (rounds of 256
+//   2. k_digit_scan    one workgroup per digit scans that digit's counts
+//                      across the tiles in place and writes its total.
+//   3. k_radix_scatter scans the 256 digit totals in LDS (the digit's first
+//                      output position), ranks keys stably inside the tile (rounds of 256
 //                      keys in input order; rank = earlier rounds + earlier
 //                      waves + earlier lanes with the same digit), stages
 //                      the tile in LDS in digit order and writes it out so
@@ -90,10 +91,12 @@ __global__ void __launch_bounds__(kSortThreads)
 __global__ void __launch_bounds__(kSortThreads)
     k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
-                    uint32_t mask, const uint32_t* __restrict__ offsets, uint32_t tiles) {
+                    uint32_t mask, const uint32_t* __restrict__ colPrefix, uint32_t tiles,
+                    const uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_wcnt[4][kRadix];
     __shared__ uint32_t s_run[kRadix];
     __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
@@ -103,6 +106,12 @@ __global__ void __launch_bounds__(kSortThreads)
     const uint64_t lt = (1ull << lane_id()) - 1ull;
     const size_t base = static_cast<size_t>(blockIdx.x) * kSortTile;
 
+    // Output position of each digit's first key: exclusive scan of the
+    // digit totals (kRadix == kSortThreads, one digit per thread).
+    {
+        uint32_t all;
+        s_base[t] = block_exclusive_scan(totals[t], s_wave, all);
+    }
     s_run[t] = 0;
     s_wcnt[0][t] = 0;
     s_wcnt[1][t] = 0;
@@ -171,7 +180,8 @@ __global__ void __launch_bounds__(kSortThreads)
         if (li < count) {
             const uint32_t kk = s_key[li];
             const uint32_t d = (kk >> shift) & mask;
-            const size_t dst = static_cast<size_t>(offsets[static_cast<size_t>(d) * tiles + blockIdx.x]) +
+            const size_t dst = static_cast<size_t>(s_base[d]) +
+                               colPrefix[static_cast<size_t>(d) * tiles + blockIdx.x] +
                                (li - s_start[d]);
             kout[dst] = kk;
             vout[dst] = s_val[li];
@@ -181,21 +191,6 @@ __global__ void __launch_bounds__(kSortThreads)
 
 // Padded LDS index for a 4096-element scan chunk (17 words per 16 values).
 __device__ __forceinline__ unsigned scan_slot(unsigned i) { return (i >> 4) * 17u + (i & 15u); }
-
-__global__ void __launch_bounds__(kSortThreads)
-    k_scan_reduce(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ partials) {
-    __shared__ uint32_t s_wave[4];
-    const size_t base = static_cast<size_t>(blockIdx.x) * kScanChunk;
-    uint32_t sum = 0;
-#pragma unroll
-    for (int r = 0; r < kScanItems; ++r) {
-        const size_t idx = base + static_cast<size_t>(r) * kSortThreads + threadIdx.x;
-        if (idx < n) sum += in[idx];
-    }
-    uint32_t total;
-    block_exclusive_scan(sum, s_wave, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
 
 // Exclusive scan of a chunk of 4096 counters in place, plus a carry-in.
 __device__ void scan_chunk(uint32_t* __restrict__ data, size_t base, size_t n, uint32_t carry,
@@ -233,26 +228,20 @@ __device__ void scan_chunk(uint32_t* __restrict__ data, size_t base, size_t n, u
     chunkTotal = total;
 }
 
-// Single-block exclusive scan of the per-chunk partial sums.
-__global__ void __launch_bounds__(kSortThreads) k_scan_partials(uint32_t* __restrict__ partials,
-                                                                size_t count) {
+// Exclusive scan of one digit's counts across the tiles, in place (one
+// workgroup per digit), and the digit's total.
+__global__ void __launch_bounds__(kSortThreads)
+    k_digit_scan(uint32_t* __restrict__ counts, uint32_t tiles, uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_buf[kScanChunk / 16 * 17];
     __shared__ uint32_t s_wave[4];
+    uint32_t* col = counts + static_cast<size_t>(blockIdx.x) * tiles;
     uint32_t carry = 0;
-    for (size_t base = 0; base < count; base += kScanChunk) {
+    for (size_t b = 0; b < tiles; b += kScanChunk) {
         uint32_t total;
-        scan_chunk(partials, base, count, carry, s_buf, s_wave, total);
+        scan_chunk(col, b, tiles, carry, s_buf, s_wave, total);
         carry += total;
     }
-}
-
-__global__ void __launch_bounds__(kSortThreads)
-    k_scan_down(uint32_t* __restrict__ data, size_t n, const uint32_t* __restrict__ partials) {
-    __shared__ uint32_t s_buf[kScanChunk / 16 * 17];
-    __shared__ uint32_t s_wave[4];
-    uint32_t total;
-    scan_chunk(data, static_cast<size_t>(blockIdx.x) * kScanChunk, n, partials[blockIdx.x], s_buf,
-               s_wave, total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // Keys with a single cell: the sort is the identity.
@@ -386,9 +375,7 @@ pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pb
 
 size_t radix_scratch_words(size_t n) {
     const size_t tiles = (n + kSortTile - 1) / kSortTile;
-    const size_t counters = tiles * kRadix;
-    const size_t chunks = (counters + kScanChunk - 1) / kScanChunk;
-    return counters + chunks + 64;
+    return tiles * kRadix + kRadix + 64;
 }
 
 int radix_key_bits(size_t numCells) {
@@ -414,9 +401,8 @@ pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* 
     const int passes = (bits + 7) / 8;
     const uint32_t tiles = static_cast<uint32_t>((n + kSortTile - 1) / kSortTile);
     const size_t counters = static_cast<size_t>(tiles) * kRadix;
-    const size_t chunks = (counters + kScanChunk - 1) / kScanChunk;
     uint32_t* counts = scratch;
-    uint32_t* partials = scratch + counters;
+    uint32_t* totals = scratch + counters;
 
     const uint32_t* srcK = kin;
     const uint32_t* srcV = vin;
@@ -429,13 +415,10 @@ pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* 
         uint32_t* dstV = toOut ? vout : tmpV;
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, srcK, n, shift,
                            mask, counts, tiles);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(chunks)), dim3(kSortThreads),
-                           0, s, counts, counters, partials);
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kSortThreads), 0, s, partials, chunks);
-        hipLaunchKernelGGL(k_scan_down, dim3(static_cast<unsigned>(chunks)), dim3(kSortThreads), 0,
-                           s, counts, counters, partials);
+        hipLaunchKernelGGL(k_digit_scan, dim3(kRadix), dim3(kSortThreads), 0, s, counts, tiles,
+                           totals);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, srcK, srcV,
-                           dstK, dstV, n, shift, mask, counts, tiles);
+                           dstK, dstV, n, shift, mask, counts, tiles, totals);
         const pbf_status st = pbf_hip_check(hipGetLastError());
         if (st != PBF_OK) return st;
         srcK = dstK;
