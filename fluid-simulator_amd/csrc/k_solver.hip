@@ -27,6 +27,39 @@ __device__ __forceinline__ size_t thread_index() {
     return static_cast<size_t>(b) * blockDim.x + threadIdx.x;
 }
 
+// Neighbour loop over a slot-major list: kNbrBatch list entries, then their
+// kNbrBatch records, are in flight before the first is used, so a wave's
+// gathers overlap instead of waiting one neighbour at a time.  Contributions
+// are still applied in list order, so every sum is formed exactly as the
+// one-neighbour-at-a-time loop forms it.  Slots past the count reload the
+// last valid entry (c >= 1 inside the loop) and are not applied.
+constexpr int kNbrBatch = 4;
+
+template <class Load, class Use>
+__device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ nbr, size_t stride,
+                                                   size_t i, int c, Load load, Use use) {
+    for (int q0 = 0; q0 < c; q0 += kNbrBatch) {
+        uint32_t j[kNbrBatch];
+#pragma unroll
+        for (int u = 0; u < kNbrBatch; ++u)
+            j[u] = nbr[static_cast<size_t>(min(q0 + u, c - 1)) * stride + i];
+        decltype(load(0u)) r[kNbrBatch];
+#pragma unroll
+        for (int u = 0; u < kNbrBatch; ++u) r[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < kNbrBatch; ++u)
+            if (q0 + u < c) use(r[u]);
+    }
+}
+
+struct PosLambda {
+    float4 p;
+    float l;
+};
+struct PosPair {
+    float4 a, b;
+};
+
 // ---------------------------------------------------------------------------
 // Reference-shaped kernels
 // ---------------------------------------------------------------------------
@@ -75,19 +108,19 @@ __global__ void __launch_bounds__(kBlock)
     // FP32 sum's significant bits.
     double rho = static_cast<double>(mass * s.poly_norm);
     double gix = 0.0, giy = 0.0, giz = 0.0, sg = 0.0;
-    for (int q = 0; q < c; ++q) {
-        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
-        const float4 pj = ld4(p, j);
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        rho += static_cast<double>(mass * poly6_d2(d2, s));
-        const float3 gw = grad<kFast>(dx, dy, dz, d2, s);
-        const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
-        sg += static_cast<double>(gjx * gjx + gjy * gjy + gjz * gjz);
-        gix -= static_cast<double>(gjx);
-        giy -= static_cast<double>(gjy);
-        giz -= static_cast<double>(gjz);
-    }
+    for_each_neighbour(
+        nbr, stride, i, c, [&](uint32_t j) { return ld4(p, j); },
+        [&](const float4& pj) {
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            rho += static_cast<double>(mass * poly6_d2(d2, s));
+            const float3 gw = grad<kFast>(dx, dy, dz, d2, s);
+            const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
+            sg += static_cast<double>(gjx * gjx + gjy * gjy + gjz * gjz);
+            gix -= static_cast<double>(gjx);
+            giy -= static_cast<double>(gjy);
+            giz -= static_cast<double>(gjz);
+        });
     sg += gix * gix + giy * giy + giz * giz;
     const double C = rho / static_cast<double>(rho0) - 1.0;
     lambda[i] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
@@ -209,17 +242,17 @@ __global__ void __launch_bounds__(kBlock)
     const float4 pi = ld4(p, i);
     const float4 vi = ld4(v, i);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-    for (int q = 0; q < c; ++q) {
-        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
-        const float4 pj = ld4(p, j);
-        const float4 vj = ld4(v, j);
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
-        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-        ox += g.y * uz - g.z * uy;
-        oy += g.z * ux - g.x * uz;
-        oz += g.x * uy - g.y * ux;
-    }
+    for_each_neighbour(
+        nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(v, j)}; },
+        [&](const PosPair& r) {
+            const float4 &pj = r.a, &vj = r.b;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            ox += g.y * uz - g.z * uy;
+            oy += g.z * ux - g.x * uz;
+            oz += g.x * uy - g.y * ux;
+        });
     const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
 }
@@ -238,23 +271,27 @@ __device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__
     const float4 wi = ld4(omega, i);
     const float wil = kFast ? wi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    for (int q = 0; q < c; ++q) {
-        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
-        const float4 pj = ld4(p, j);
-        float wjl;
-        if (kFast) {
-            wjl = omega[j].w;
-        } else {
-            const float4 wj = ld4(omega, j);
-            wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
-        }
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
-        const float md = wjl - wil;
-        ex += md * g.x;
-        ey += md * g.y;
-        ez += md * g.z;
-    }
+    for_each_neighbour(
+        nbr, stride, i, c,
+        [&](uint32_t j) {
+            float wjl;
+            if (kFast) {
+                wjl = omega[j].w;
+            } else {
+                const float4 wj = ld4(omega, j);
+                wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
+            }
+            return PosLambda{ld4(p, j), wjl};
+        },
+        [&](const PosLambda& r) {
+            const float4& pj = r.p;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+            const float md = r.l - wil;
+            ex += md * g.x;
+            ey += md * g.y;
+            ez += md * g.z;
+        });
     const float el = __builtin_sqrtf(ex * ex + ey * ey + ez * ez);
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     if (el > 1.0e-6f) {  // normalizationEpsilon, pbf_solver_kernels.cu:367
@@ -474,17 +511,18 @@ __global__ void __launch_bounds__(kBlock)
         const Scorr sc = scorr_consts(h, sk, sn, sdq);
         const float li = lambda[i];
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for (int q = 0; q < c; ++q) {
-            const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
-            const float4 pj = ld4(pin, j);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const float3 gw = grad<true>(dx, dy, dz, d2, s);
-            const float ls = li + lambda[j] + scorr_value(d2, s.h2, sc);
-            ax += ls * gw.x;
-            ay += ls * gw.y;
-            az += ls * gw.z;
-        }
+        for_each_neighbour(
+            nbr, stride, i, c, [&](uint32_t j) { return PosLambda{ld4(pin, j), lambda[j]}; },
+            [&](const PosLambda& r) {
+                const float4& pj = r.p;
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float3 gw = grad<true>(dx, dy, dz, d2, s);
+                const float ls = li + r.l + scorr_value(d2, s.h2, sc);
+                ax += ls * gw.x;
+                ay += ls * gw.y;
+                az += ls * gw.z;
+            });
         const float cs = mass / rho0;
         dpx = cs * ax;
         dpy = cs * ay;
@@ -525,19 +563,19 @@ __global__ void __launch_bounds__(kBlock)
     }
     const Sph s = sph_consts(h);
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for (int q = 0; q < c; ++q) {
-        const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
-        const float4 pj = ld4(p, j);
-        const float4 xj = ld4(x, j);
-        const float vjx = (pj.x - xj.x) * inv_dt;
-        const float vjy = (pj.y - xj.y) * inv_dt;
-        const float vjz = (pj.z - xj.z) * inv_dt;
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
-        cx += (vjx - vix) * w;
-        cy += (vjy - viy) * w;
-        cz += (vjz - viz) * w;
-    }
+    for_each_neighbour(
+        nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
+        [&](const PosPair& r) {
+            const float4 &pj = r.a, &xj = r.b;
+            const float vjx = (pj.x - xj.x) * inv_dt;
+            const float vjy = (pj.y - xj.y) * inv_dt;
+            const float vjz = (pj.z - xj.z) * inv_dt;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
+            cx += (vjx - vix) * w;
+            cy += (vjy - viy) * w;
+            cz += (vjz - viz) * w;
+        });
     st4(vout, i, make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw));
 }
 
