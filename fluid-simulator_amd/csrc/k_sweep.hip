@@ -72,15 +72,12 @@ struct Meta {
     int tileFirst[kThreads];  // their first particles
 };
 
-// Staged coordinates of one z-plane, one array per component: lanes of a
-// wave sit in ~8 neighbouring cells and read records ~8 apart at every
-// step, which 16-byte records would map onto one bank group; 4-byte
-// components spread them over the banks, and the three loads of a
-// candidate share one address register with immediate offsets.
+// Staged coordinates of one z-plane as packed 12-byte (x, y, z) records:
+// the twelve loads of a four-candidate step share one address register with
+// immediate offsets, and lanes ~8 records apart (neighbouring cells) land
+// 24 banks apart, where 16-byte records would share one bank group.
 struct Plane {
-    float x[kCap + kPad];
-    float y[kCap + kPad];
-    float z[kCap + kPad];
+    float xyz[3 * (kCap + kPad)];
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
@@ -112,8 +109,8 @@ struct LdsRow {
     const Plane* pl;
     int base;
     __device__ float3 position(int q) const {
-        const int k = base + q;
-        return make_float3(pl->x[k], pl->y[k], pl->z[k]);
+        const float* r = pl->xyz + 3 * (base + q);
+        return make_float3(r[0], r[1], r[2]);
     }
 };
 
@@ -127,6 +124,28 @@ struct GlobalRow {
     }
 };
 
+// mask = 2 * mask + (d[u] <= h2) for u = 0, 1, 2, 3 in turn: one compare
+// into a scalar pair and one add-with-carry per candidate (the compiler's
+// select, shift and or take three).  The four compares come first, so each
+// carry is read three instructions after it is written, past the two wait
+// states a VALU-written scalar needs before a VALU reads it.  The compare is
+// h2 >= d2, false for NaN exactly as d2 <= h2.
+__device__ __forceinline__ uint32_t push4(uint32_t mask, float h2, float d0, float d1, float d2,
+                                          float d3) {
+    uint64_t c0, c1, c2, c3, co;
+    asm("v_cmp_ge_f32_e64 %1, %6, %7\n\t"
+        "v_cmp_ge_f32_e64 %2, %6, %8\n\t"
+        "v_cmp_ge_f32_e64 %3, %6, %9\n\t"
+        "v_cmp_ge_f32_e64 %4, %6, %10\n\t"
+        "v_addc_co_u32_e64 %0, %5, %0, %0, %1\n\t"
+        "v_addc_co_u32_e64 %0, %5, %0, %0, %2\n\t"
+        "v_addc_co_u32_e64 %0, %5, %0, %0, %3\n\t"
+        "v_addc_co_u32_e64 %0, %5, %0, %0, %4"
+        : "+v"(mask), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=&s"(co)
+        : "s"(h2), "v"(d0), "v"(d1), "v"(d2), "v"(d3));
+    return mask;
+}
+
 // Appends the neighbours of particle gi among candidates [lo, hi) of one
 // row (first = the row's first working-set index) to gi's list, in order.
 template <class Src>
@@ -134,17 +153,20 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
                                           int selfq, size_t gi, size_t first, int& count) {
     for (int c = lo; c < hi; c += 32) {
         const int w = min(32, hi - c);
+        const int w4 = (w + 3) & ~3;
         uint32_t mask = 0u;
         for (int b = 0; b < w; b += 4) {
             float3 j[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) j[u] = src.position(c + b + u);
+            float d[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float d2 = dist2_exact(ip.x - j[u].x, ip.y - j[u].y, ip.z - j[u].z);
-                mask |= static_cast<uint32_t>(d2 <= g.h2) << (b + u);
-            }
+            for (int u = 0; u < 4; ++u)
+                d[u] = dist2_exact(ip.x - j[u].x, ip.y - j[u].y, ip.z - j[u].z);
+            mask = push4(mask, g.h2, d[0], d[1], d[2], d[3]);
         }
+        // Candidate b sits at bit w4 - 1 - b; reverse so that it sits at b.
+        mask = __builtin_bitreverse32(mask) >> (32 - w4);
         if (w < 32) mask &= (1u << w) - 1u;
         if (selfq >= c && selfq < c + 32) mask &= ~(1u << (selfq - c));
         while (mask != 0u) {
@@ -253,9 +275,10 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
                     const int base = m.segBase[s];
                     for (int q = tid; q < cnt; q += kThreads) {
                         const float4 a = ld4(g.p, first + q);
-                        lds.x[base + q] = a.x;
-                        lds.y[base + q] = a.y;
-                        lds.z[base + q] = a.z;
+                        float* r = lds.xyz + 3 * (base + q);
+                        r[0] = a.x;
+                        r[1] = a.y;
+                        r[2] = a.z;
                     }
                 }
                 lds_barrier();
