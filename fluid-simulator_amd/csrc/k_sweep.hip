@@ -169,13 +169,27 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
         mask = __builtin_bitreverse32(mask) >> (32 - w4);
         if (w < 32) mask &= (1u << w) - 1u;
         if (selfq >= c && selfq < c + 32) mask &= ~(1u << (selfq - c));
+        if (mask == 0u) continue;
+        // Entries past maxNeighbors are counted, not stored: keep the
+        // lowest free-slot-many bits (rare), so the loop below stores all.
+        const int hits = __builtin_popcount(mask);
+        const int room = g.maxNeighbors - count;
+        if (hits > room) {
+            uint32_t kept = 0u;
+            for (int k = 0; k < room; ++k) {
+                const uint32_t low = mask & (0u - mask);
+                kept |= low;
+                mask ^= low;
+            }
+            mask = kept;
+        }
+        uint32_t* out = g.nbr + static_cast<size_t>(count) * g.stride + gi;
+        const uint32_t base = static_cast<uint32_t>(first) + static_cast<uint32_t>(c);
+        count += hits;
         while (mask != 0u) {
-            const int b = __builtin_ctz(mask);
+            *out = base + static_cast<uint32_t>(__builtin_ctz(mask));
             mask &= mask - 1u;
-            if (count < g.maxNeighbors)
-                g.nbr[static_cast<size_t>(count) * g.stride + gi] =
-                    static_cast<uint32_t>(first + static_cast<size_t>(c + b));
-            ++count;
+            out += g.stride;
         }
     }
 }
@@ -344,8 +358,12 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
 }
 
 // One workgroup per 256 sorted particles: it sweeps every tile whose first
-// particle falls in its range (none, one, or several small ones).
-__global__ void __launch_bounds__(kThreads) k_build_lists(Grid g) {
+// particle falls in its range (none, one, or several small ones).  Seven
+// waves per SIMD (72 VGPRs, three spilled on cold paths) rather than the six
+// that 77 VGPRs allow: the sweep waits on LDS and the list stores most of
+// the time, and the seventh wave hides part of it.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
+k_build_lists(Grid g) {
     __shared__ Meta m;
     __shared__ Plane lds;
 
