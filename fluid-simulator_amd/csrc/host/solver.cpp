@@ -8,11 +8,12 @@
 //   integrate + v_in snapshot + first cell keys      -> k_predict_key
 //   per iteration: grid build, reorder               -> radix sort, ranges, k_gather
 //                  neighbour lists                    -> k_build_lists (LDS cell sweep)
-//                  lambda                             -> k_lambda_list
+//                  lambda (into p*.w)                 -> k_lambda_list
 //                  delta-p, apply, collision solve    -> k_delta_apply_collide
+//                  (restores p*.w from x.w)
 //   post-solve rebuild (if XSPH or vorticity)         -> as above
 //   velocity update + XSPH                            -> k_update_xsph
-//   vorticity                                         -> k_vorticity_list
+//   vorticity (+ (x, |omega|) records)                -> k_vorticity_list
 //   confinement + velocity contacts                   -> k_confine_resolve
 // The lists are the reference's (slot-major, same neighbours in the same
 // order); the list kernels run their fast form (hardware rsq in the spiky
@@ -154,7 +155,6 @@ struct PBFSolver::State {
     DeviceBuffer<int> cnt;
     std::size_t stride = 0;
     DeviceBuffer<std::uint32_t> tileEnd;  // sweep tile ends of the last build
-    DeviceBuffer<float> lambda;
 
     hipStream_t stream = nullptr;
     Profiler prof;
@@ -244,13 +244,12 @@ struct PBFSolver::State {
                     Scope sc(prof, "lambda", stream);
                     launch_check(pbfint::launch_lambda_guarded(
                         p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
-                        P.restDensity, P.lambdaEpsilon, lambda.data(), stride, status.data(),
-                        stream));
+                        P.restDensity, P.lambdaEpsilon, stride, status.data(), stream));
                 }
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
                     launch_check(pbfint::launch_delta_apply_collide(
-                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, lambda.data(),
+                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
                         stride, col, P.particleRadius, status.data(), stream));
                 }
@@ -276,15 +275,17 @@ struct PBFSolver::State {
             if (P.vorticityStrength != 0.0f) {
                 {
                     Scope sc(prof, "vorticity", stream);
+                    // p holds the previous x after the swap and is free until
+                    // the next substep's prediction: it takes the records.
                     launch_check(pbfint::launch_vorticity(x.data(), v.data(), nbr.data(),
                                                           cnt.data(), kMaxNeighbors, n, h,
-                                                          omega.data(), stride, stream));
+                                                          omega.data(), p.data(), stride, stream));
                 }
                 {
                     Scope sc(prof, "confine_resolve", stream);
                     launch_check(pbfint::launch_confine_resolve(
-                        x.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(), v.data(),
-                        vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
+                        x.data(), p.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(),
+                        v.data(), vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
                         P.particleRadius, P.collisionRestitution, P.collisionFriction,
                         status.data(), stream));
                 }
@@ -398,7 +399,6 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
         b->allocate(maxParticles);
     s.id.allocate(maxParticles);
     s.ida.allocate(maxParticles);
-    s.lambda.allocate(maxParticles);
     s.status.allocate(1);
     s.nbr.allocate(stride * static_cast<std::size_t>(kMaxNeighbors));
     s.cnt.allocate(maxParticles);

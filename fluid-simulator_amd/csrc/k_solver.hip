@@ -85,12 +85,17 @@ __global__ void __launch_bounds__(kBlock) k_integrate(const pbf_float4* __restri
     st4(p, i, pi);
 }
 
-// computeLambda (src/pbf/pbf_solver_kernels.cu:26-105).
+// computeLambda (src/pbf/pbf_solver_kernels.cu:26-105).  lambda_i is stored
+// at lambda[i * lstride]: the C ABI passes its own array (lstride 1), the
+// step passes the .w words of p itself (lstride 4), so that Delta-p reads a
+// neighbour's position and lambda in one 16-byte gather.  Storing .w while
+// other lanes read .xyz of the same records touches disjoint bytes, hence
+// neither pointer is restrict-qualified.
 template <bool kFast>
 __global__ void __launch_bounds__(kBlock)
-    k_lambda_list(const pbf_float4* __restrict__ p, const uint32_t* __restrict__ nbr,
+    k_lambda_list(const pbf_float4* p, const uint32_t* __restrict__ nbr,
                   const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
-                  float eps, float* __restrict__ lambda, size_t stride,
+                  float eps, float* lambda, size_t lstride, size_t stride,
                   const uint32_t* __restrict__ status) {
     const size_t i = thread_index<kFast>();
     if (i >= n) return;
@@ -98,7 +103,7 @@ __global__ void __launch_bounds__(kBlock)
     const float4 pi = ld4(p, i);
     const int c = cnt[i];
     if (!valid_count(c, maxN)) {
-        lambda[i] = qnan();
+        lambda[i * lstride] = qnan();
         return;
     }
     const Sph s = sph_consts(h);
@@ -123,7 +128,7 @@ __global__ void __launch_bounds__(kBlock)
         });
     sg += gix * gix + giy * giy + giz * giz;
     const double C = rho / static_cast<double>(rho0) - 1.0;
-    lambda[i] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
+    lambda[i * lstride] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
 }
 
 // computeDeltaPosition (src/pbf/pbf_solver_kernels.cu:108-182).
@@ -224,22 +229,25 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // computeVorticity (src/pbf/pbf_solver_kernels.cu:268-315).  The kFast form
-// also stores |omega| in .w (the reference writes 0 there), which the
-// step's confinement kernel reads instead of recomputing it per neighbour.
+// also stores |omega| in .w (the reference writes 0 there) and the record
+// (p.xyz, |omega|) in pw, so that the step's confinement kernel reads a
+// neighbour's position and |omega| in one 16-byte gather.
 template <bool kFast>
 __global__ void __launch_bounds__(kBlock)
     k_vorticity_list(const pbf_float4* __restrict__ p, const pbf_float4* __restrict__ v,
                      const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
-                     size_t n, float h, pbf_float4* __restrict__ omega, size_t stride) {
+                     size_t n, float h, pbf_float4* __restrict__ omega,
+                     pbf_float4* __restrict__ pw, size_t stride) {
     const size_t i = thread_index<kFast>();
     if (i >= n) return;
     const int c = cnt[i];
+    const float4 pi = ld4(p, i);
     if (!valid_count(c, maxN)) {
         st4(omega, i, make_float4(qnan(), qnan(), qnan(), 0.0f));
+        if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, qnan()));
         return;
     }
     const Sph s = sph_consts(h);
-    const float4 pi = ld4(p, i);
     const float4 vi = ld4(v, i);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
     for_each_neighbour(
@@ -255,33 +263,35 @@ __global__ void __launch_bounds__(kBlock)
         });
     const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
+    if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, wl));
 }
 
 // applyVorticityConfinement (src/pbf/pbf_solver_kernels.cu:317-385) as a
 // device function, shared by the API kernel and the fused confinement +
-// contact kernel.  The kFast form takes |omega| from .w (written by the
-// kFast vorticity kernel).  Returns false for an invalid neighbour count.
+// contact kernel.  The kFast form reads p = the (position, |omega|) records
+// of the kFast vorticity kernel: one gather per neighbour.
 template <bool kFast>
-__device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__ p,
+__device__ __forceinline__ void confine(size_t i, const pbf_float4* __restrict__ p,
                                         const uint32_t* __restrict__ nbr, int c,
                                         const pbf_float4* __restrict__ omega, float4 vi,
                                         const Sph& s, float dt, float strength, size_t stride,
                                         float4& out) {
     const float4 pi = ld4(p, i);
     const float4 wi = ld4(omega, i);
-    const float wil = kFast ? wi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
+    const float wil = kFast ? pi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     for_each_neighbour(
         nbr, stride, i, c,
         [&](uint32_t j) {
+            const float4 pj = ld4(p, j);
             float wjl;
             if (kFast) {
-                wjl = omega[j].w;
+                wjl = pj.w;
             } else {
                 const float4 wj = ld4(omega, j);
                 wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
             }
-            return PosLambda{ld4(p, j), wjl};
+            return PosLambda{pj, wjl};
         },
         [&](const PosLambda& r) {
             const float4& pj = r.p;
@@ -303,7 +313,6 @@ __device__ __forceinline__ bool confine(size_t i, const pbf_float4* __restrict__
     const float fy = strength * (nz * wi.x - nx * wi.z);
     const float fz = strength * (nx * wi.y - ny * wi.x);
     out = make_float4(vi.x + dt * fx, vi.y + dt * fy, vi.z + dt * fz, vi.w);
-    return true;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -491,11 +500,13 @@ __global__ void __launch_bounds__(kBlock)
 // computeDeltaPosition + applyDeltaPosition + CollisionSystem::solve for one
 // constraint iteration (pbf_solver.cu:240-257).  Reads p* from pin and writes
 // the corrected, projected p* to pout (Jacobi), so every thread sees the
-// positions the reference's separate Delta-p kernel sees.
+// positions the reference's separate Delta-p kernel sees.  pin[].w holds
+// lambda (stored there by the lambda pass); pout[].w gets back the
+// particle's own .w, which x carries.
 __global__ void __launch_bounds__(kBlock)
     k_delta_apply_collide(const pbf_float4* __restrict__ pin, pbf_float4* __restrict__ pout,
                           const uint32_t* __restrict__ nbr, const int* __restrict__ cnt,
-                          int maxN, const float* __restrict__ lambda, size_t n, float h,
+                          int maxN, const pbf_float4* __restrict__ x, size_t n, float h,
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
                           Colliders col, float radius, uint32_t* __restrict__ status) {
     const size_t i = thread_index<true>();
@@ -509,16 +520,15 @@ __global__ void __launch_bounds__(kBlock)
     } else {
         const Sph s = sph_consts(h);
         const Scorr sc = scorr_consts(h, sk, sn, sdq);
-        const float li = lambda[i];
+        const float li = pi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         for_each_neighbour(
-            nbr, stride, i, c, [&](uint32_t j) { return PosLambda{ld4(pin, j), lambda[j]}; },
-            [&](const PosLambda& r) {
-                const float4& pj = r.p;
+            nbr, stride, i, c, [&](uint32_t j) { return ld4(pin, j); },
+            [&](const float4& pj) {
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 const float d2 = dx * dx + dy * dy + dz * dz;
                 const float3 gw = grad<true>(dx, dy, dz, d2, s);
-                const float ls = li + r.l + scorr_value(d2, s.h2, sc);
+                const float ls = li + pj.w + scorr_value(d2, s.h2, sc);
                 ax += ls * gw.x;
                 ay += ls * gw.y;
                 az += ls * gw.z;
@@ -531,6 +541,7 @@ __global__ void __launch_bounds__(kBlock)
     pi.x += dpx;
     pi.y += dpy;
     pi.z += dpz;
+    pi.w = x[i].w;
     if (!collision_fixed_point(pi, col, radius))
         atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
     st4(pout, i, pi);
@@ -597,8 +608,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // applyVorticityConfinement + resolveVelocitiesKernel (pbf_solver.cu:309-323).
+// pw holds the vorticity pass's (position, |omega|) records.
 __global__ void __launch_bounds__(kBlock)
-    k_confine_resolve(const pbf_float4* __restrict__ x, const uint32_t* __restrict__ nbr,
+    k_confine_resolve(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ pw,
+                      const uint32_t* __restrict__ nbr,
                       const int* __restrict__ cnt, int maxN, const pbf_float4* __restrict__ omega,
                       const pbf_float4* __restrict__ vcur, const pbf_float4* __restrict__ vsnap,
                       pbf_float4* __restrict__ vout, size_t n, float h, float dt, float strength,
@@ -614,7 +627,7 @@ __global__ void __launch_bounds__(kBlock)
         out = make_float4(qnan(), qnan(), qnan(), vi.w);
     } else {
         const Sph s = sph_consts(h);
-        confine<true>(i, x, nbr, c, omega, vi, s, dt, strength, stride, out);
+        confine<true>(i, pw, nbr, c, omega, vi, s, dt, strength, stride, out);
     }
     resolve_velocity(ld4(x, i), ld4(vsnap, i), out, col, radius, e, f);
     st4(vout, i, out);
@@ -670,7 +683,7 @@ pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const 
                                 float* lambda, size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_lambda_list<false>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, stride, static_cast<const uint32_t*>(nullptr));
+               rho0, eps, lambda, size_t{1}, stride, static_cast<const uint32_t*>(nullptr));
 }
 
 pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
@@ -706,7 +719,7 @@ pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, con
                                    size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_vorticity_list<false>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, stride);
+               omega, static_cast<pbf_float4*>(nullptr), stride);
 }
 
 pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
@@ -822,21 +835,21 @@ pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 
     PBF_LAUNCH(k_keys, grid_for(n), kBlock, stream, p, key, n, mn, cs, gs);
 }
 
-pbf_status launch_lambda_guarded(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
-                                 int maxN, size_t n, float h, float mass, float rho0, float eps,
-                                 float* lambda, size_t stride, const uint32_t* status,
-                                 void* stream) {
+pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
+                                 size_t n, float h, float mass, float rho0, float eps,
+                                 size_t stride, const uint32_t* status, void* stream) {
+    float* lambda = reinterpret_cast<float*>(p) + 3;  // the .w words
     PBF_LAUNCH(k_lambda_list<true>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, stride, status);
+               rho0, eps, lambda, size_t{4}, stride, status);
 }
 
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
-                                      const int* cnt, int maxN, const float* lambda, size_t n,
+                                      const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const Colliders& col, float radius,
                                       uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_delta_apply_collide, grid_for(n), kBlock, stream, pin, pout, nbr, cnt, maxN,
-               lambda, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, status);
+    PBF_LAUNCH(k_delta_apply_collide, grid_for(n), kBlock, stream, pin, pout, nbr, cnt, maxN, x,
+               n, h, mass, rho0, sk, sn, sdq, stride, col, radius, status);
 }
 
 pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
@@ -855,18 +868,18 @@ pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p,
 
 pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
-                            size_t stride, void* stream) {
+                            pbf_float4* pw, size_t stride, void* stream) {
     PBF_LAUNCH(k_vorticity_list<true>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, stride);
+               omega, pw, stride);
 }
 
-pbf_status launch_confine_resolve(const pbf_float4* x, const uint32_t* nbr, const int* cnt,
-                                  int maxN, const pbf_float4* omega, const pbf_float4* vcur,
-                                  const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
-                                  float dt, float strength, size_t stride, const Colliders& col,
-                                  float radius, float e, float f, const uint32_t* status,
-                                  void* stream) {
-    PBF_LAUNCH(k_confine_resolve, grid_for(n), kBlock, stream, x, nbr, cnt, maxN, omega, vcur,
+pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,
+                                  const int* cnt, int maxN, const pbf_float4* omega,
+                                  const pbf_float4* vcur, const pbf_float4* vsnap, pbf_float4* vout,
+                                  size_t n, float h, float dt, float strength, size_t stride,
+                                  const Colliders& col, float radius, float e, float f,
+                                  const uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_confine_resolve, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN, omega, vcur,
                vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, status);
 }
 

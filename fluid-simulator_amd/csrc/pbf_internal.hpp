@@ -37,12 +37,13 @@ pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v,
                               float cs, pbf_int3 gs, const uint32_t* status, void* stream);
 pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
                        pbf_int3 gs, void* stream);
-pbf_status launch_lambda_guarded(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
-                                 int maxN, size_t n, float h, float mass, float rho0, float eps,
-                                 float* lambda, size_t stride, const uint32_t* status,
-                                 void* stream);
+// lambda goes to p[].w (the step's Delta-p reads it there).
+pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
+                                 size_t n, float h, float mass, float rho0, float eps,
+                                 size_t stride, const uint32_t* status, void* stream);
+// Reads lambda from pin[].w and restores pout[].w from x[].w.
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
-                                      const int* cnt, int maxN, const float* lambda, size_t n,
+                                      const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const pbfdev::Colliders& col, float radius,
                                       uint32_t* status, void* stream);
@@ -53,11 +54,13 @@ pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pb
 pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
                                        size_t n, float inv_dt, const uint32_t* status,
                                        void* stream);
+// Writes omega (with |omega| in .w) and the records pw = (p.xyz, |omega|).
 pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
-                            size_t stride, void* stream);
-pbf_status launch_confine_resolve(const pbf_float4* x, const uint32_t* nbr, const int* cnt,
-                                  int maxN, const pbf_float4* omega, const pbf_float4* vcur,
+                            pbf_float4* pw, size_t stride, void* stream);
+pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,
+                                  const int* cnt, int maxN, const pbf_float4* omega,
+                                  const pbf_float4* vcur,
                                   const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
                                   float dt, float strength, size_t stride,
                                   const pbfdev::Colliders& col, float radius, float e, float f,
