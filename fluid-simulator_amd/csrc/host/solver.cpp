@@ -15,9 +15,11 @@
 //   velocity update + XSPH                            -> k_update_xsph
 //   vorticity (+ (x, |omega|) records)                -> k_vorticity_list
 //   confinement + velocity contacts                   -> k_confine_resolve
-// The lists are the reference's (slot-major, same neighbours in the same
-// order); the list kernels run their fast form (hardware rsq in the spiky
-// gradient, workgroups contiguous per XCD).
+// The lists hold the reference's neighbours in the reference's order, stored
+// in groups of four entries (one 16-byte access per four neighbours) rather
+// than slot-major; copyNeighborLists hands them out slot-major.  The list
+// kernels run their fast form (hardware rsq in the spiky gradient,
+// workgroups contiguous per XCD).
 // Neighbour overflow and collision non-convergence set bits of a device
 // status word; every later kernel that changes the state returns at once,
 // and the host reads the word once when the step (or run) finishes.
@@ -27,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "host_internal.hpp"
 #include "pbf/pbf_solver.hpp"
@@ -544,10 +547,19 @@ void PBFSolver::copyNeighborLists(float4* positions, int* counts, std::uint32_t*
     hip_check(hipStreamSynchronize(s.stream));
     hip_check(hipMemcpy(positions, s.x.data(), n * sizeof(float4), hipMemcpyDeviceToHost));
     hip_check(hipMemcpy(counts, s.cnt.data(), n * sizeof(int), hipMemcpyDeviceToHost));
-    if (maxNeighbors > 0)
-        hip_check(hipMemcpy2D(neighbors, n * sizeof(std::uint32_t), s.nbr.data(),
-                              s.stride * sizeof(std::uint32_t), n * sizeof(std::uint32_t),
-                              static_cast<std::size_t>(maxNeighbors), hipMemcpyDeviceToHost));
+    if (maxNeighbors > 0) {
+        // The step keeps its lists in groups of four (entry q of particle i
+        // at word 4 * ((q / 4) * stride + i) + q % 4); hand them out
+        // slot-major, as findNeighbors writes them.
+        const std::size_t groups = (static_cast<std::size_t>(maxNeighbors) + 3) / 4;
+        std::vector<std::uint32_t> g(groups * n * 4);
+        hip_check(hipMemcpy2D(g.data(), n * 4 * sizeof(std::uint32_t), s.nbr.data(),
+                              s.stride * 4 * sizeof(std::uint32_t), n * 4 * sizeof(std::uint32_t),
+                              groups, hipMemcpyDeviceToHost));
+        for (std::size_t q = 0; q < static_cast<std::size_t>(maxNeighbors); ++q)
+            for (std::size_t i = 0; i < n; ++i)
+                neighbors[q * n + i] = g[((q / 4) * n + i) * 4 + q % 4];
+    }
 }
 
 }  // namespace pbf

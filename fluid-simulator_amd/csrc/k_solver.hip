@@ -27,22 +27,34 @@ __device__ __forceinline__ size_t thread_index() {
     return static_cast<size_t>(b) * blockDim.x + threadIdx.x;
 }
 
-// Neighbour loop over a slot-major list: kNbrBatch list entries, then their
-// kNbrBatch records, are in flight before the first is used, so a wave's
-// gathers overlap instead of waiting one neighbour at a time.  Contributions
-// are still applied in list order, so every sum is formed exactly as the
-// one-neighbour-at-a-time loop forms it.  Slots past the count reload the
-// last valid entry (c >= 1 inside the loop) and are not applied.
-constexpr int kNbrBatch = 4;
+// Neighbour loop: kNbrBatch list entries, then their kNbrBatch records, are
+// in flight before the first is used, so a wave's gathers overlap instead of
+// waiting one neighbour at a time.  Contributions are still applied in list
+// order, so every sum is formed exactly as the one-neighbour-at-a-time loop
+// forms it.  kGrouped reads the step's grouped lists (four entries per
+// 16-byte load, free slots of the last group repeating its last entry);
+// otherwise the reference's slot-major list, where slots past the count
+// reload the last valid entry (c >= 1 inside the loop).  Slots past the
+// count are not applied.
+constexpr int kNbrBatch = 4;  // a group of the step's lists
 
-template <class Load, class Use>
+template <bool kGrouped, class Load, class Use>
 __device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ nbr, size_t stride,
                                                    size_t i, int c, Load load, Use use) {
     for (int q0 = 0; q0 < c; q0 += kNbrBatch) {
         uint32_t j[kNbrBatch];
+        if (kGrouped) {
+            const uint4* groups = reinterpret_cast<const uint4*>(nbr);
+            const uint4 g = groups[static_cast<size_t>(q0 / 4) * stride + i];
+            j[0] = g.x;
+            j[1] = g.y;
+            j[2] = g.z;
+            j[3] = g.w;
+        } else {
 #pragma unroll
-        for (int u = 0; u < kNbrBatch; ++u)
-            j[u] = nbr[static_cast<size_t>(min(q0 + u, c - 1)) * stride + i];
+            for (int u = 0; u < kNbrBatch; ++u)
+                j[u] = nbr[static_cast<size_t>(min(q0 + u, c - 1)) * stride + i];
+        }
         decltype(load(0u)) r[kNbrBatch];
 #pragma unroll
         for (int u = 0; u < kNbrBatch; ++u) r[u] = load(j[u]);
@@ -113,7 +125,7 @@ __global__ void __launch_bounds__(kBlock)
     // FP32 sum's significant bits.
     double rho = static_cast<double>(mass * s.poly_norm);
     double gix = 0.0, giy = 0.0, giz = 0.0, sg = 0.0;
-    for_each_neighbour(
+    for_each_neighbour<kFast>(
         nbr, stride, i, c, [&](uint32_t j) { return ld4(p, j); },
         [&](const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -250,7 +262,7 @@ __global__ void __launch_bounds__(kBlock)
     const Sph s = sph_consts(h);
     const float4 vi = ld4(v, i);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-    for_each_neighbour(
+    for_each_neighbour<kFast>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(v, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &vj = r.b;
@@ -280,7 +292,7 @@ __device__ __forceinline__ void confine(size_t i, const pbf_float4* __restrict__
     const float4 wi = ld4(omega, i);
     const float wil = kFast ? pi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    for_each_neighbour(
+    for_each_neighbour<kFast>(
         nbr, stride, i, c,
         [&](uint32_t j) {
             const float4 pj = ld4(p, j);
@@ -522,7 +534,7 @@ __global__ void __launch_bounds__(kBlock)
         const Scorr sc = scorr_consts(h, sk, sn, sdq);
         const float li = pi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_each_neighbour(
+        for_each_neighbour<true>(
             nbr, stride, i, c, [&](uint32_t j) { return ld4(pin, j); },
             [&](const float4& pj) {
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -574,7 +586,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     const Sph s = sph_consts(h);
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for_each_neighbour(
+    for_each_neighbour<true>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &xj = r.b;

@@ -20,6 +20,11 @@
 //   * each thread owns one particle of the tile.  For every row it tests the
 //     candidates of cells cx-1 .. cx+1 in windows of 32 (branch-free, four
 //     loads in flight) and appends the set bits, lowest first, to its list;
+//   * lists are stored in groups of four entries, group k of particle i at
+//     nbr[k * stride + i] (16 bytes), so one store carries four entries and
+//     the list kernels read four entries with one load.  Slots of the last
+//     group past the count repeat its last entry, so every slot of a
+//     written group is a valid index;
 //   * a workgroup whose range starts more than kMaxTilePasses tiles (sparse
 //     regions: spray, surface sheets) gives each of their particles a lane
 //     of its own that reads its 27 cells from global memory instead of
@@ -49,7 +54,7 @@ struct Grid {
     const int* cellEnd;
     const uint32_t* sortedKeys;
     const uint32_t* tileEnd;
-    uint32_t* nbr;
+    uint4* nbr;  // grouped lists
     int* cnt;
     uint32_t* status;
     size_t n;
@@ -146,11 +151,20 @@ __device__ __forceinline__ uint32_t push4(uint32_t mask, float h2, float d0, flo
     return mask;
 }
 
+// A particle's list under construction.  buf holds the open group of four,
+// newest entry in .w; count is the number of neighbours found (it may pass
+// maxNeighbors), stored the number kept (at most maxNeighbors).
+struct ListState {
+    uint4 buf;
+    int count;
+    int stored;
+};
+
 // Appends the neighbours of particle gi among candidates [lo, hi) of one
 // row (first = the row's first working-set index) to gi's list, in order.
 template <class Src>
 __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 ip, int lo, int hi,
-                                          int selfq, size_t gi, size_t first, int& count) {
+                                          int selfq, size_t gi, size_t first, ListState& st) {
     for (int c = lo; c < hi; c += 32) {
         const int w = min(32, hi - c);
         const int w4 = (w + 3) & ~3;
@@ -171,9 +185,10 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
         if (selfq >= c && selfq < c + 32) mask &= ~(1u << (selfq - c));
         if (mask == 0u) continue;
         // Entries past maxNeighbors are counted, not stored: keep the
-        // lowest free-slot-many bits (rare), so the loop below stores all.
+        // lowest free-slot-many bits (rare), so the loop below keeps all.
         const int hits = __builtin_popcount(mask);
-        const int room = g.maxNeighbors - count;
+        const int room = g.maxNeighbors - st.stored;
+        st.count += hits;
         if (hits > room) {
             uint32_t kept = 0u;
             for (int k = 0; k < room; ++k) {
@@ -183,20 +198,33 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
             }
             mask = kept;
         }
-        uint32_t* out = g.nbr + static_cast<size_t>(count) * g.stride + gi;
+        uint4* out = g.nbr + static_cast<size_t>(st.stored >> 2) * g.stride + gi;
         const uint32_t base = static_cast<uint32_t>(first) + static_cast<uint32_t>(c);
-        count += hits;
         while (mask != 0u) {
-            *out = base + static_cast<uint32_t>(__builtin_ctz(mask));
+            const uint32_t e = base + static_cast<uint32_t>(__builtin_ctz(mask));
             mask &= mask - 1u;
-            out += g.stride;
+            st.buf = make_uint4(st.buf.y, st.buf.z, st.buf.w, e);
+            if ((++st.stored & 3) == 0) {
+                *out = st.buf;
+                out += g.stride;
+            }
         }
     }
 }
 
-__device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, int count) {
-    g.cnt[gi] = count;
-    if (count > g.maxNeighbors) atomicOr(g.status, PBF_STATUS_NEIGHBOR_OVERFLOW);
+// Stores the open group, its free slots repeating the last entry, and the
+// count.
+__device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const ListState& st) {
+    const int k = st.stored & 3;
+    if (k != 0) {
+        const uint4 b = st.buf;
+        const uint4 last = k == 1   ? make_uint4(b.w, b.w, b.w, b.w)
+                           : k == 2 ? make_uint4(b.z, b.w, b.w, b.w)
+                                    : make_uint4(b.y, b.z, b.w, b.w);
+        g.nbr[static_cast<size_t>(st.stored >> 2) * g.stride + gi] = last;
+    }
+    g.cnt[gi] = st.count;
+    if (st.count > g.maxNeighbors) atomicOr(g.status, PBF_STATUS_NEIGHBOR_OVERFLOW);
 }
 
 // One tile: segment geometry, then the tile's particles against the three
@@ -269,7 +297,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
         const size_t gi = own0 + static_cast<size_t>(active ? r : c0);
         float3 ip = make_float3(0.0f, 0.0f, 0.0f);
         int mc = 0;
-        int count = 0;
+        ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
         if (active) {
             const float4 p = ld4(g.p, gi);
             ip = make_float3(p.x, p.y, p.z);
@@ -309,14 +337,14 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
                 const size_t first = static_cast<size_t>(m.segFirst[s]);
                 if (staged) {
                     const LdsRow src{&lds, m.segBase[s]};
-                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, count);
+                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
                 } else {
                     const GlobalRow src{g.p, first, m.cellOff[s][nc] - 1};
-                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, count);
+                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
                 }
             }
         }
-        if (active) finish_particle(g, gi, count);
+        if (active) finish_particle(g, gi, st);
     }
     lds_barrier();  // LDS is reused by the next tile
 }
@@ -333,7 +361,7 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
     const int cz = static_cast<int>(rowId / static_cast<uint32_t>(g.dims.y));
     const float4 p = ld4(g.p, gi);
     const float3 ip = make_float3(p.x, p.y, p.z);
-    int count = 0;
+    ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
     const int xa = max(cx - 1, 0);
     const int xb = min(cx + 1, g.dims.x - 1);
     for (int s = 0; s < kSegs; ++s) {
@@ -352,9 +380,9 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
         const int len = rowEnd - rowStart;
         const int selfq = s == 4 ? static_cast<int>(gi - static_cast<size_t>(rowStart)) : -1;
         const GlobalRow src{g.p, static_cast<size_t>(rowStart), len - 1};
-        sweep_row(g, src, ip, 0, len, selfq, gi, static_cast<size_t>(rowStart), count);
+        sweep_row(g, src, ip, 0, len, selfq, gi, static_cast<size_t>(rowStart), st);
     }
-    finish_particle(g, gi, count);
+    finish_particle(g, gi, st);
 }
 
 // One workgroup per 256 sorted particles: it sweeps every tile whose first
@@ -427,13 +455,17 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
     if (sg.n == 0) return PBF_OK;
     if (stride < sg.n)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "neighbour list stride below particle count");
+    if (sg.maxNeighbors < 0 || sg.maxNeighbors % 4 != 0)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "grouped lists need a multiple of 4 slots");
+    if (reinterpret_cast<uintptr_t>(nbr) % alignof(uint4) != 0)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "grouped lists need 16-byte alignment");
     Grid g;
     g.p = p;
     g.cellStart = sg.cellStart;
     g.cellEnd = sg.cellEnd;
     g.sortedKeys = sg.sortedKeys;
     g.tileEnd = sg.tileEnd;
-    g.nbr = nbr;
+    g.nbr = reinterpret_cast<uint4*>(nbr);
     g.cnt = cnt;
     g.status = status;
     g.n = sg.n;

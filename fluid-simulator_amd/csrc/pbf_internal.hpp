@@ -89,9 +89,12 @@ struct SweepGrid {
 };
 // Entries of the tileEnd array for a grid of these dims.
 size_t sweep_tile_count(pbf_int3 dims);
-// The list launch_find_neighbors builds (nbr[q * stride + i], at most
-// maxNeighbors stored, counts whole), bit for bit; returns at once when
-// *status is non-zero and raises PBF_STATUS_NEIGHBOR_OVERFLOW.
+// The lists launch_find_neighbors builds (at most maxNeighbors stored,
+// counts whole), bit for bit, in grouped layout: entry q of particle i at
+// word 4 * ((q / 4) * stride + i) + q % 4, and the slots of the last group
+// past the count repeat its last entry (maxNeighbors a multiple of 4, nbr
+// 16-byte aligned).  Returns at once when *status is non-zero and raises
+// PBF_STATUS_NEIGHBOR_OVERFLOW.
 pbf_status sweep_build_lists(const SweepGrid& g, const pbf_float4* p, uint32_t* nbr, int* cnt,
                              size_t stride, uint32_t* status, void* stream);
 // CollisionSystem::solve as one launch: per-particle fixed point, at most 32
