@@ -245,6 +245,11 @@ pbf_status pbf_solver_counts(const pbf_solver* s, size_t* count, size_t* maxPart
 
 void* pbf_solver_stream(pbf_solver* s) { return s ? s->solver.stream() : nullptr; }
 
+pbf_status pbf_solver_set_graphs(pbf_solver* s, int enabled) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setGraphs(enabled != 0); });
+}
+
 pbf_status pbf_solver_set_profiling(pbf_solver* s, int enabled) {
     PBF_REQUIRE(s);
     return guarded([&] { s->solver.setProfiling(enabled != 0); });

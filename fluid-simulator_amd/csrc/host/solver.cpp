@@ -23,6 +23,7 @@
 // Neighbour overflow and collision non-convergence set bits of a device
 // status word; every later kernel that changes the state returns at once,
 // and the host reads the word once when the step (or run) finishes.
+#include <array>
 #include <cmath>
 #include <limits>
 #include <memory>
@@ -162,8 +163,103 @@ struct PBFSolver::State {
     hipStream_t stream = nullptr;
     Profiler prof;
 
+    // Frames replayed from HIP graphs.  A frame permutes the working-set
+    // buffers (every grid build swaps them with their alternates, the
+    // velocity update swaps x and p*), and a graph bakes in the pointers it
+    // was captured with, so one graph is kept per starting assignment,
+    // with the permutation the frame applies; a workload cycles through one
+    // or two of them.  Kernel arguments are baked in too: initialize,
+    // setParticles and the collider setters drop the graphs.
+    using BufferKey = std::array<const void*, 10>;
+    struct FrameGraph {
+        BufferKey start{};
+        std::array<int, 10> perm{};  // slot k ends up holding slot perm[k]'s buffer
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<FrameGraph> graphs;
+    bool useGraphs = false;
+
     ~State() {
+        drop_graphs();
         if (stream != nullptr) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graphs() noexcept {
+        for (FrameGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+
+    // Slots 0-7: x, xa, p, pa, v, va, vin, vina; 8-9: id, ida.
+    BufferKey buffer_key() const {
+        return {x.data(), xa.data(), p.data(), pa.data(), v.data(), va.data(),
+                vin.data(), vina.data(), id.data(), ida.data()};
+    }
+
+    void permute(const std::array<int, 10>& perm) {
+        std::array<DeviceBuffer<float4>*, 8> f{&x, &xa, &p, &pa, &v, &va, &vin, &vina};
+        std::array<DeviceBuffer<std::uint32_t>*, 2> u{&id, &ida};
+        std::array<DeviceBuffer<float4>, 8> tf;
+        std::array<DeviceBuffer<std::uint32_t>, 2> tu;
+        for (int k = 0; k < 8; ++k) tf[k] = std::move(*f[perm[k]]);
+        for (int k = 0; k < 2; ++k) tu[k] = std::move(*u[perm[8 + k] - 8]);
+        for (int k = 0; k < 8; ++k) *f[k] = std::move(tf[k]);
+        for (int k = 0; k < 2; ++k) *u[k] = std::move(tu[k]);
+    }
+
+    // The permutation that takes assignment `from` to assignment `to`.
+    static std::array<int, 10> perm_between(const BufferKey& from, const BufferKey& to) {
+        std::array<int, 10> perm{};
+        for (int k = 0; k < 10; ++k) {
+            const int lo = k < 8 ? 0 : 8, hi = k < 8 ? 8 : 10;
+            perm[k] = -1;
+            for (int j = lo; j < hi; ++j)
+                if (from[j] == to[k]) perm[k] = j;
+            if (perm[k] < 0) throw std::logic_error("Working-set buffers lost track of a slot");
+        }
+        return perm;
+    }
+
+    // One frame on the stream: the captured graph of this buffer
+    // assignment, captured now if there is none; launched eagerly when
+    // graphs are off or the step is being profiled (events per launch).
+    void frame() {
+        if (!useGraphs || prof.enabled) {
+            enqueue_frame();
+            return;
+        }
+        const BufferKey key = buffer_key();
+        for (const FrameGraph& g : graphs) {
+            if (g.start == key) {
+                hip_check(hipGraphLaunch(g.exec, stream));
+                permute(g.perm);
+                return;
+            }
+        }
+        FrameGraph fg;
+        fg.start = key;
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_frame();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph != nullptr) (void)hipGraphDestroy(graph);
+            permute(perm_between(buffer_key(), key));
+            throw;
+        }
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        fg.perm = perm_between(key, buffer_key());
+        permute(perm_between(buffer_key(), key));  // nothing ran yet
+        if (ce != hipSuccess) {
+            if (graph != nullptr) (void)hipGraphDestroy(graph);
+            hip_check(ce);
+        }
+        const hipError_t ie = hipGraphInstantiate(&fg.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(ie);
+        graphs.push_back(fg);
+        hip_check(hipGraphLaunch(fg.exec, stream));
+        permute(fg.perm);
     }
 
     void ensure_stream() {
@@ -389,6 +485,7 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
 
     State& s = *state_;
     GridCore grid = GridCore::create(maxParticles, minBounds, maxBounds, p.smoothingRadius);
+    s.drop_graphs();
 
     // From here a failed allocation leaves the solver inactive
     // (pbf_solver.cu:84-87).
@@ -424,6 +521,7 @@ void PBFSolver::setParticles(const float4* positions, const float4* velocities, 
     }
     if (positions == nullptr || velocities == nullptr)
         throw std::invalid_argument("Particle positions and velocities must not be null");
+    s.drop_graphs();
     for (std::size_t i = 0; i < n; ++i) {
         if (!finite4(positions[i]) || !finite4(velocities[i]))
             throw std::invalid_argument("Particle coordinates and velocities must be finite");
@@ -449,6 +547,7 @@ void PBFSolver::setParticlesDevice(const float4* positions, const float4* veloci
     }
     if (positions == nullptr || velocities == nullptr)
         throw std::invalid_argument("Particle positions and velocities must not be null");
+    s.drop_graphs();
     hip_check(hipDeviceSynchronize());
     hip_check(hipMemcpyAsync(s.x.data(), positions, n * sizeof(float4), hipMemcpyDeviceToDevice,
                              s.stream));
@@ -461,12 +560,15 @@ void PBFSolver::setParticlesDevice(const float4* positions, const float4* veloci
 
 void PBFSolver::setSpheres(const std::vector<SphereCollider>& spheres) {
     state_->collision.setSpheres(spheres);
+    state_->drop_graphs();
 }
 void PBFSolver::setBoxes(const std::vector<BoxCollider>& boxes) {
     state_->collision.setBoxes(boxes);
+    state_->drop_graphs();
 }
 void PBFSolver::setPlanes(const std::vector<PlaneCollider>& planes) {
     state_->collision.setPlanes(planes);
+    state_->drop_graphs();
 }
 
 void PBFSolver::step() {
@@ -474,7 +576,7 @@ void PBFSolver::step() {
     if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before stepping");
     if (s.count == 0) return;
     s.reset_status();
-    s.enqueue_frame();
+    s.frame();
     s.finish();
 }
 
@@ -489,7 +591,7 @@ void PBFSolver::run(std::size_t frameCount) {
     if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before running");
     if (s.count == 0 || frameCount == 0) return;
     s.reset_status();
-    for (std::size_t f = 0; f < frameCount; ++f) s.enqueue_frame();
+    for (std::size_t f = 0; f < frameCount; ++f) s.frame();
     s.finish();
 }
 
@@ -524,6 +626,11 @@ void PBFSolver::copyVelocitiesToDevice(float4* out, std::size_t n) const {
 std::size_t PBFSolver::particleCount() const { return state_->count; }
 std::size_t PBFSolver::maxParticles() const { return state_->maxParticles; }
 void* PBFSolver::stream() const { return state_->stream; }
+
+void PBFSolver::setGraphs(bool enabled) {
+    state_->useGraphs = enabled;
+    if (!enabled) state_->drop_graphs();
+}
 
 void PBFSolver::setProfiling(bool enabled) {
     state_->prof.enabled = enabled;

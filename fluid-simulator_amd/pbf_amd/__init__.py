@@ -242,6 +242,7 @@ _SIGNATURES = {
     "pbf_solver_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
     "pbf_solver_stream": (_vp, [_vp]),
     "pbf_solver_set_profiling": (_st, [_vp, _i]),
+    "pbf_solver_set_graphs": (_st, [_vp, _i]),
     "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
     "pbf_solver_neighbor_lists": (_st, [_vp, _vp, _vp, _vp, _sz, _i]),
@@ -584,6 +585,10 @@ class PBFSolver:
 
     def stream(self) -> int:
         return lib().pbf_solver_stream(self._h) or 0
+
+    def setGraphs(self, enabled: bool):
+        """Replay frames from captured HIP graphs, or launch eagerly (default)."""
+        _check(lib().pbf_solver_set_graphs(self._h, 1 if enabled else 0))
 
     def setProfiling(self, enabled: bool):
         _check(lib().pbf_solver_set_profiling(self._h, 1 if enabled else 0))

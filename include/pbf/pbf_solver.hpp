@@ -55,6 +55,9 @@ public:
 
     // Per-kernel event timing of the step (see pbf_solver_set_profiling).
     void setProfiling(bool enabled);
+    // Replay frames from captured HIP graphs (default off; see
+    // pbf_solver_set_graphs).  No counterpart in the reference.
+    void setGraphs(bool enabled);
     struct KernelTime {
         std::string name;
         double milliseconds;

@@ -354,6 +354,16 @@ pbf_status pbf_solver_counts(const pbf_solver* solver, size_t* particleCount,
 /* The stream the solver enqueues on (hipStream_t as void*). */
 void* pbf_solver_stream(pbf_solver* solver);
 
+/* Frames of pbf_solver_step / pbf_solver_run replayed from HIP graphs
+ * (enabled = 1) or launched kernel by kernel (0, the default).  No
+ * counterpart in the reference, whose step() launches every kernel from the
+ * host and synchronises about 15 times: the status word already removed the
+ * syncs, and a graph per frame removes the per-launch host work.  Results
+ * are bitwise the same either way.  Graphs are captured on first use and
+ * dropped by initialize, set_particles and the collider setters; frames run
+ * eagerly while profiling is on. */
+pbf_status pbf_solver_set_graphs(pbf_solver* solver, int enabled);
+
 /* Per-kernel timing on the solver's stream with HIP events.  While enabled,
  * every launch of the step is bracketed by events; pbf_solver_kernel_times
  * returns accumulated milliseconds and launch counts per kernel name.
