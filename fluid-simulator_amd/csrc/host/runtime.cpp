@@ -67,6 +67,18 @@ pbf_status pbf_device_free(void* ptr) {
     return pbf_hip_check(hipFree(ptr));
 }
 
+pbf_status pbf_host_alloc(void** ptr, size_t bytes) {
+    if (ptr == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null output pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return PBF_OK;
+    return pbf_hip_check(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+}
+
+pbf_status pbf_host_free(void* ptr) {
+    if (ptr == nullptr) return PBF_OK;
+    return pbf_hip_check(hipHostFree(ptr));
+}
+
 pbf_status pbf_memcpy_htod(void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return PBF_OK;
     if (dst == nullptr || src == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null pointer");

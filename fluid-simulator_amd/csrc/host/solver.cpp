@@ -23,8 +23,10 @@
 // Neighbour overflow and collision non-convergence set bits of a device
 // status word; every later kernel that changes the state returns at once,
 // and the host reads the word once when the step (or run) finishes.
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -144,6 +146,59 @@ bool finite4(const float4& v) {
     return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z);
 }
 
+// Pinned host staging for uploads and copy-outs through pageable memory:
+// two chunk buffers used in turn, so the host-side copy (and validation) of
+// one chunk overlaps the DMA of the other.  An event per buffer marks the
+// end of the last DMA that used it.
+class PinnedStage {
+public:
+    static constexpr std::size_t kChunk = std::size_t{1} << 19;  // float4 per chunk (8 MiB)
+
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage() {
+        for (int k = 0; k < 2; ++k) {
+            if (buf_[k] != nullptr) (void)hipHostFree(buf_[k]);
+            if (done_[k] != nullptr) (void)hipEventDestroy(done_[k]);
+        }
+    }
+
+    float4* buffer(int k) {
+        ensure();
+        return buf_[k];
+    }
+    hipEvent_t event(int k) {
+        ensure();
+        return done_[k];
+    }
+
+private:
+    void ensure() {
+        if (buf_[0] != nullptr) return;
+        for (int k = 0; k < 2; ++k) {
+            void* p = nullptr;
+            hip_check(hipHostMalloc(&p, kChunk * sizeof(float4), hipHostMallocDefault));
+            buf_[k] = static_cast<float4*>(p);
+            hip_check(hipEventCreateWithFlags(&done_[k], hipEventDisableTiming));
+        }
+    }
+
+    float4* buf_[2] = {nullptr, nullptr};
+    hipEvent_t done_[2] = {nullptr, nullptr};
+};
+
+// True for memory the device can reach directly (pinned host or device).
+bool device_visible(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory reports an error
+        return false;
+    }
+    return a.type == hipMemoryTypeHost || a.type == hipMemoryTypeDevice ||
+           a.type == hipMemoryTypeManaged;
+}
+
 }  // namespace
 
 struct PBFSolver::State {
@@ -162,6 +217,7 @@ struct PBFSolver::State {
 
     hipStream_t stream = nullptr;
     Profiler prof;
+    PinnedStage stage;
 
     // Frames replayed from HIP graphs.  A frame permutes the working-set
     // buffers (every grid build swaps them with their alternates, the
@@ -400,6 +456,73 @@ struct PBFSolver::State {
 
     void reset_status() { launch_check(pbfint::launch_zero_u32(status.data(), 1, stream)); }
 
+    // Host -> device copy of n records, each record checked first: one DMA
+    // from pinned sources, else through the pinned stage (the check and the
+    // staging copy are one pass over the source, overlapping the previous
+    // chunk's DMA).  Throws, with the stream drained, at the first
+    // non-finite coordinate.
+    void upload_checked(const float4* src, float4* dst, std::size_t n) {
+        if (device_visible(src)) {
+            for (std::size_t k = 0; k < n; ++k)
+                if (!finite4(src[k]))
+                    throw std::invalid_argument(
+                        "Particle coordinates and velocities must be finite");
+            hip_check(hipMemcpyAsync(dst, src, n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            return;
+        }
+        const std::size_t per = PinnedStage::kChunk;
+        int slot = 0;
+        try {
+            for (std::size_t i0 = 0; i0 < n; i0 += per, slot ^= 1) {
+                const std::size_t m = std::min(per, n - i0);
+                hip_check(hipEventSynchronize(stage.event(slot)));
+                float4* h = stage.buffer(slot);
+                for (std::size_t k = 0; k < m; ++k) {
+                    const float4 v = src[i0 + k];
+                    if (!finite4(v))
+                        throw std::invalid_argument(
+                            "Particle coordinates and velocities must be finite");
+                    h[k] = v;
+                }
+                hip_check(hipMemcpyAsync(dst + i0, h, m * sizeof(float4), hipMemcpyHostToDevice,
+                                         stream));
+                hip_check(hipEventRecord(stage.event(slot), stream));
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            throw;
+        }
+    }
+
+    // Device -> host copy of n records: straight into pinned destinations,
+    // else through the pinned stage, the host copy of one chunk overlapping
+    // the DMA of the next.  Returns with the copy complete.
+    void download(const float4* src, float4* out, std::size_t n) {
+        if (device_visible(out)) {
+            hip_check(hipMemcpyAsync(out, src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
+            return;
+        }
+        const std::size_t per = PinnedStage::kChunk;
+        const std::size_t chunks = (n + per - 1) / per;
+        for (std::size_t c = 0; c <= chunks; ++c) {
+            if (c < chunks) {
+                const std::size_t i0 = c * per;
+                const std::size_t m = std::min(per, n - i0);
+                const int slot = static_cast<int>(c & 1);
+                hip_check(hipMemcpyAsync(stage.buffer(slot), src + i0, m * sizeof(float4),
+                                         hipMemcpyDeviceToHost, stream));
+                hip_check(hipEventRecord(stage.event(slot), stream));
+            }
+            if (c > 0) {
+                const std::size_t i0 = (c - 1) * per;
+                const std::size_t m = std::min(per, n - i0);
+                const int slot = static_cast<int>((c - 1) & 1);
+                hip_check(hipEventSynchronize(stage.event(slot)));
+                std::memcpy(out + i0, stage.buffer(slot), m * sizeof(float4));
+            }
+        }
+    }
+
     // Waits for the device and raises what the status word recorded.
     void finish() {
         hip_check(hipStreamSynchronize(stream));
@@ -421,8 +544,7 @@ struct PBFSolver::State {
                 hip_check(hipStreamSynchronize(stream));
                 throw std::invalid_argument("Destination cannot be null");
             }
-            hip_check(hipMemcpyAsync(out, scratch.data(), n * sizeof(float4), hipMemcpyDeviceToHost,
-                                     stream));
+            download(scratch.data(), out, n);
         } else {
             if (out == nullptr) {
                 hip_check(hipStreamSynchronize(stream));
@@ -522,16 +644,15 @@ void PBFSolver::setParticles(const float4* positions, const float4* velocities, 
     if (positions == nullptr || velocities == nullptr)
         throw std::invalid_argument("Particle positions and velocities must not be null");
     s.drop_graphs();
-    for (std::size_t i = 0; i < n; ++i) {
-        if (!finite4(positions[i]) || !finite4(velocities[i]))
-            throw std::invalid_argument("Particle coordinates and velocities must be finite");
-    }
-    hip_check(hipMemcpyAsync(s.x.data(), positions, n * sizeof(float4), hipMemcpyHostToDevice,
-                             s.stream));
-    hip_check(hipMemcpyAsync(s.v.data(), velocities, n * sizeof(float4), hipMemcpyHostToDevice,
-                             s.stream));
+    // Validated while staged into the alternate buffers, which hold nothing
+    // between steps: a non-finite value throws with x and v untouched
+    // (pbf_solver.cu:131-143 validates both arrays before uploading).
+    s.upload_checked(positions, s.xa.data(), n);
+    s.upload_checked(velocities, s.va.data(), n);
     launch_check(pbf_k_initialize_stable_particle_ids(s.id.data(), n, s.stream));
     hip_check(hipStreamSynchronize(s.stream));
+    std::swap(s.x, s.xa);
+    std::swap(s.v, s.va);
     s.count = n;
 }
 

@@ -171,6 +171,8 @@ _SIGNATURES = {
     "pbf_version": (C.c_char_p, []),
     "pbf_device_alloc": (_st, [C.POINTER(_vp), _sz]),
     "pbf_device_free": (_st, [_vp]),
+    "pbf_host_alloc": (_st, [C.POINTER(_vp), _sz]),
+    "pbf_host_free": (_st, [_vp]),
     "pbf_memcpy_htod": (_st, [_vp, _vp, _sz]),
     "pbf_memcpy_dtoh": (_st, [_vp, _vp, _sz]),
     "pbf_memcpy_dtod": (_st, [_vp, _vp, _sz]),
@@ -299,6 +301,33 @@ def as_float4(a, n=None) -> np.ndarray:
         raise InvalidArgument("float4 arrays must have shape (n, 4)")
     return arr
 
+
+# --------------------------------------------------------------------------
+# Pinned host memory
+# --------------------------------------------------------------------------
+class _PinnedBlock:
+    def __init__(self, nbytes: int):
+        p = _vp()
+        _check(lib().pbf_host_alloc(C.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pbf_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in pinned (page-locked) host memory: the solver's
+    uploads from and copy-outs into it run as one DMA each.  The allocation
+    is freed with the last array that views it."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(count * dtype.itemsize, 1)
+    block = _PinnedBlock(nbytes)
+    buf = (C.c_char * nbytes).from_address(block.ptr)
+    buf._block = block  # the buffer, and every array over it, keeps the block alive
+    return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
 
 # --------------------------------------------------------------------------
 # Device memory

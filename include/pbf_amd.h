@@ -108,6 +108,11 @@ pbf_status pbf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 /* Memset semantics: every byte gets the low 8 bits of byteValue
  * (CudaBuffer::fillBytes, cuda_buffer.hpp:67-77). */
 pbf_status pbf_memset(void* d_dst, int byteValue, size_t bytes);
+/* Pinned (page-locked) host memory.  No counterpart in the reference: the
+ * solver's uploads from and copy-outs into such memory run as one DMA each
+ * instead of going through the solver's pinned staging chunks. */
+pbf_status pbf_host_alloc(void** h_ptr, size_t bytes);
+pbf_status pbf_host_free(void* h_ptr);
 pbf_status pbf_device_synchronize(void);
 pbf_status pbf_stream_create(void** stream);
 pbf_status pbf_stream_destroy(void* stream);
