@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "host_internal.hpp"
+#include "pbf/pbf_batch.hpp"
 #include "pbf/pbf_solver.hpp"
 #include "pbf_amd.h"
 #include "pbf_launch.hpp"
@@ -17,6 +18,9 @@ struct pbf_collision {
 };
 struct pbf_solver {
     pbf::PBFSolver solver;
+};
+struct pbf_batch {
+    pbf::PBFBatchSolver batch;
 };
 
 namespace {
@@ -57,6 +61,30 @@ template <typename T>
 std::vector<T> to_vector(const T* data, size_t count) {
     if (count != 0 && data == nullptr) throw std::invalid_argument("null collider array");
     return std::vector<T>(data, data + count);
+}
+
+}  // namespace
+
+namespace {
+
+void write_kernel_times(const std::vector<pbf::PBFSolver::KernelTime>& times, char* names,
+                        size_t namesBytes, double* ms, uint64_t* launches, size_t capacity,
+                        size_t* kernelCount) {
+    if (kernelCount) *kernelCount = times.size();
+    std::string joined;
+    for (size_t i = 0; i < times.size(); ++i) {
+        if (i < capacity) {
+            if (ms) ms[i] = times[i].milliseconds;
+            if (launches) launches[i] = times[i].launches;
+        }
+        joined += times[i].name;
+        joined += '\n';
+    }
+    if (names != nullptr && namesBytes > 0) {
+        const size_t len = joined.size() < namesBytes - 1 ? joined.size() : namesBytes - 1;
+        std::memcpy(names, joined.data(), len);
+        names[len] = '\0';
+    }
 }
 
 }  // namespace
@@ -259,22 +287,8 @@ pbf_status pbf_solver_kernel_times(pbf_solver* s, char* names, size_t namesBytes
                                    uint64_t* launches, size_t capacity, size_t* kernelCount) {
     PBF_REQUIRE(s);
     return guarded([&] {
-        const auto times = s->solver.kernelTimes();
-        if (kernelCount) *kernelCount = times.size();
-        std::string joined;
-        for (size_t i = 0; i < times.size(); ++i) {
-            if (i < capacity) {
-                if (ms) ms[i] = times[i].milliseconds;
-                if (launches) launches[i] = times[i].launches;
-            }
-            joined += times[i].name;
-            joined += '\n';
-        }
-        if (names != nullptr && namesBytes > 0) {
-            const size_t len = joined.size() < namesBytes - 1 ? joined.size() : namesBytes - 1;
-            std::memcpy(names, joined.data(), len);
-            names[len] = '\0';
-        }
+        write_kernel_times(s->solver.kernelTimes(), names, namesBytes, ms, launches, capacity,
+                           kernelCount);
     });
 }
 
@@ -283,6 +297,82 @@ pbf_status pbf_solver_neighbor_lists(pbf_solver* s, pbf_float4* positions, int* 
     PBF_REQUIRE(s);
     return guarded([&] {
         s->solver.copyNeighborLists(positions, counts, neighbors, n, maxNeighbors);
+    });
+}
+
+pbf_status pbf_batch_create(pbf_batch** batch) {
+    if (batch == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null output pointer");
+    return guarded([&] { *batch = new pbf_batch(); });
+}
+
+pbf_status pbf_batch_destroy(pbf_batch* batch) {
+    delete batch;
+    return PBF_OK;
+}
+
+pbf_status pbf_batch_initialize(pbf_batch* b, const pbf_batch_scene* scenes, size_t sceneCount,
+                                const pbf_sim_params* params) {
+    PBF_REQUIRE(b);
+    if (params == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null parameters");
+    if (scenes == nullptr && sceneCount != 0)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null scene table");
+    return guarded([&] {
+        std::vector<pbf::BatchScene> v(sceneCount);
+        for (size_t k = 0; k < sceneCount; ++k)
+            v[k] = pbf::BatchScene{scenes[k].particleCount, scenes[k].minBounds, scenes[k].maxBounds};
+        b->batch.initialize(v, *params);
+    });
+}
+
+pbf_status pbf_batch_set_particles(pbf_batch* b, const pbf_float4* pos, const pbf_float4* vel,
+                                   size_t n) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setParticles(pos, vel, n); });
+}
+
+pbf_status pbf_batch_step(pbf_batch* b) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.step(); });
+}
+
+pbf_status pbf_batch_run(pbf_batch* b, size_t frames) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.run(frames); });
+}
+
+pbf_status pbf_batch_copy_positions_to_host(pbf_batch* b, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.copyPositionsToHost(out, n); });
+}
+
+pbf_status pbf_batch_copy_velocities_to_host(pbf_batch* b, pbf_float4* out, size_t n) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.copyVelocitiesToHost(out, n); });
+}
+
+pbf_status pbf_batch_counts(const pbf_batch* b, size_t* sceneCount, size_t* particleCount) {
+    PBF_REQUIRE(b);
+    if (sceneCount) *sceneCount = b->batch.sceneCount();
+    if (particleCount) *particleCount = b->batch.particleCount();
+    return PBF_OK;
+}
+
+pbf_status pbf_batch_set_graphs(pbf_batch* b, int enabled) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setGraphs(enabled != 0); });
+}
+
+pbf_status pbf_batch_set_profiling(pbf_batch* b, int enabled) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setProfiling(enabled != 0); });
+}
+
+pbf_status pbf_batch_kernel_times(pbf_batch* b, char* names, size_t namesBytes, double* ms,
+                                  uint64_t* launches, size_t capacity, size_t* kernelCount) {
+    PBF_REQUIRE(b);
+    return guarded([&] {
+        write_kernel_times(b->batch.kernelTimes(), names, namesBytes, ms, launches, capacity,
+                           kernelCount);
     });
 }
 

@@ -35,26 +35,7 @@ GridCore GridCore::create(std::size_t maxParticles, float3 minBounds, float3 max
     next.maxBounds = maxBounds;
     next.cellSize = cellSize;
 
-    // ceil((max - min) / cellSize) in double (spatial_grid.cu:37-53).
-    const auto axis = [cellSize](float lo, float hi) {
-        const double count = std::ceil((static_cast<double>(hi) - static_cast<double>(lo)) /
-                                       static_cast<double>(cellSize));
-        if (!std::isfinite(count) || count < 1.0 ||
-            count > static_cast<double>(std::numeric_limits<int>::max()))
-            throw std::length_error("Spatial grid axis exceeds integer capacity");
-        return static_cast<int>(count);
-    };
-    next.dims.x = axis(minBounds.x, maxBounds.x);
-    next.dims.y = axis(minBounds.y, maxBounds.y);
-    next.dims.z = axis(minBounds.z, maxBounds.z);
-
-    const std::size_t maxCells = static_cast<std::size_t>(std::numeric_limits<std::uint32_t>::max());
-    const std::size_t cx = static_cast<std::size_t>(next.dims.x);
-    const std::size_t cy = static_cast<std::size_t>(next.dims.y);
-    const std::size_t cz = static_cast<std::size_t>(next.dims.z);
-    if (cx > maxCells / cy || cx * cy > maxCells / cz)
-        throw std::length_error("Spatial grid exceeds 32-bit key capacity");
-    next.numCells = cx * cy * cz;
+    next.dims = dimsFor(minBounds, maxBounds, cellSize, &next.numCells);
     next.keyBits = pbfint::radix_key_bits(next.numCells);
 
     next.keys.allocate(maxParticles);
@@ -67,14 +48,55 @@ GridCore GridCore::create(std::size_t maxParticles, float3 minBounds, float3 max
     return next;
 }
 
+int3 GridCore::dimsFor(float3 minBounds, float3 maxBounds, float cellSize, std::size_t* cells) {
+    // ceil((max - min) / cellSize) in double (spatial_grid.cu:37-53).
+    const auto axis = [cellSize](float lo, float hi) {
+        const double count = std::ceil((static_cast<double>(hi) - static_cast<double>(lo)) /
+                                       static_cast<double>(cellSize));
+        if (!std::isfinite(count) || count < 1.0 ||
+            count > static_cast<double>(std::numeric_limits<int>::max()))
+            throw std::length_error("Spatial grid axis exceeds integer capacity");
+        return static_cast<int>(count);
+    };
+    int3 d;
+    d.x = axis(minBounds.x, maxBounds.x);
+    d.y = axis(minBounds.y, maxBounds.y);
+    d.z = axis(minBounds.z, maxBounds.z);
+    const std::size_t maxCells = static_cast<std::size_t>(std::numeric_limits<std::uint32_t>::max());
+    const std::size_t cx = static_cast<std::size_t>(d.x);
+    const std::size_t cy = static_cast<std::size_t>(d.y);
+    const std::size_t cz = static_cast<std::size_t>(d.z);
+    if (cx > maxCells / cy || cx * cy > maxCells / cz)
+        throw std::length_error("Spatial grid exceeds 32-bit key capacity");
+    *cells = cx * cy * cz;
+    return d;
+}
+
+GridCore GridCore::createKeyed(std::size_t maxParticles, std::size_t numCells, float cellSize) {
+    if (numCells == 0 || numCells > static_cast<std::size_t>(std::numeric_limits<std::uint32_t>::max()))
+        throw std::length_error("Batched grids exceed 32-bit key capacity");
+    GridCore next;
+    next.maxParticles = maxParticles;
+    next.cellSize = cellSize;
+    next.numCells = numCells;
+    next.keyBits = pbfint::radix_key_bits(numCells);
+    next.keys.allocate(maxParticles);
+    next.indices.allocate(maxParticles);
+    next.sortedKeys.allocate(maxParticles);
+    next.sortedIndices.allocate(maxParticles);
+    next.cellStart.allocate(numCells);
+    next.cellEnd.allocate(numCells);
+    next.scratch.allocate(pbfint::radix_scratch_words(maxParticles));
+    return next;
+}
+
 void GridCore::sortAndRanges(std::size_t n, hipStream_t stream) {
     launch_check(pbfint::radix_sort_pairs(keys.data(), nullptr, sortedKeys.data(),
                                           sortedIndices.data(), keys.data(), indices.data(), n,
                                           keyBits, scratch.data(), stream));
     launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
     launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,
-                                            dims, nullptr,
-                                            stream));
+                                            dims, nullptr, pbfint::SceneTable{}, stream));
 }
 
 SpatialGrid::SpatialGrid() : state_(new State()) {}
@@ -104,7 +126,7 @@ void SpatialGrid::build(const float4* predicted, std::size_t n) {
         return;
     }
     launch_check(pbfint::launch_keys(predicted, g.keys.data(), n, g.minBounds, g.cellSize, g.dims,
-                                     stream));
+                                     pbfint::SceneTable{}, stream));
     g.sortAndRanges(n, stream);
     // Surface asynchronous failures from this call (spatial_grid.cu:220-225).
     hip_check(hipDeviceSynchronize());

@@ -42,6 +42,12 @@ struct GridCore {
     // Validates like the reference and allocates; throws on failure.
     static GridCore create(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
                            float cellSize);
+    // Cells per axis over [minBounds, maxBounds] and their count, validated
+    // like create (which uses it).
+    static int3 dimsFor(float3 minBounds, float3 maxBounds, float cellSize, std::size_t* cells);
+    // The keyed part of a grid for a batch of scenes: numCells global cells
+    // (the scenes' grids laid end to end), no bounds of its own.
+    static GridCore createKeyed(std::size_t maxParticles, std::size_t numCells, float cellSize);
 
     // Stable sort of `keys` (slot values implicit) and the cell ranges.
     void sortAndRanges(std::size_t particleCount, hipStream_t stream);

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "host_internal.hpp"
+#include "pbf/pbf_batch.hpp"
 #include "pbf/pbf_solver.hpp"
 #include "pbf_internal.hpp"
 
@@ -215,6 +216,21 @@ struct PBFSolver::State {
     std::size_t stride = 0;
     DeviceBuffer<std::uint32_t> tileEnd;  // sweep tile ends of the last build
 
+    // Scenes of a batched solver (PBFBatchSolver); none for one scene.  Each
+    // scene keeps its grid, container and slot range in device tables.
+    std::size_t sceneCount = 0;
+    std::vector<std::size_t> sceneFirst;  // sceneCount + 1 slot offsets
+    DeviceBuffer<pbfint::SceneGeom> sceneGeom;
+    DeviceBuffer<pbf_container> sceneContainers;
+    DeviceBuffer<std::uint32_t> sceneOf;
+
+    pbfint::SceneTable scenes() const {
+        if (sceneCount == 0) return pbfint::SceneTable{};
+        return pbfint::SceneTable{sceneGeom.data(), sceneContainers.data(), sceneOf.data()};
+    }
+
+    void allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles);
+
     hipStream_t stream = nullptr;
     Profiler prof;
     PinnedStage stage;
@@ -331,7 +347,7 @@ struct PBFSolver::State {
         if (!keysReady) {
             Scope sc(prof, "keys", stream);
             launch_check(pbfint::launch_keys(p.data(), grid.keys.data(), n, grid.minBounds,
-                                             grid.cellSize, grid.dims, stream));
+                                             grid.cellSize, grid.dims, scenes(), stream));
         }
         {
             Scope sc(prof, "radix_sort", stream);
@@ -347,7 +363,7 @@ struct PBFSolver::State {
                                                    grid.numCells, stream));
             launch_check(pbfint::launch_cell_ranges(grid.sortedKeys.data(), grid.cellStart.data(),
                                                     grid.cellEnd.data(), n, grid.dims,
-                                                    tileEnd.data(), stream));
+                                                    tileEnd.data(), scenes(), stream));
         }
         {
             Scope sc(prof, "gather", stream);
@@ -366,7 +382,7 @@ struct PBFSolver::State {
     pbfint::SweepGrid sweep_grid() {
         return pbfint::SweepGrid{grid.cellStart.data(), grid.cellEnd.data(),
                                  grid.sortedKeys.data(), tileEnd.data(), grid.dims, count,
-                                 params.smoothingRadius, kMaxNeighbors};
+                                 params.smoothingRadius, kMaxNeighbors, scenes()};
     }
 
     // Neighbour lists by cell sweep, then the flat list kernels.
@@ -390,7 +406,7 @@ struct PBFSolver::State {
                 launch_check(pbfint::launch_predict_key(
                     x.data(), p.data(), v.data(), vin.data(), grid.keys.data(),
                     static_cast<int>(n), sdt, P.gravity, grid.minBounds, grid.cellSize, grid.dims,
-                    status.data(), stream));
+                    scenes(), status.data(), stream));
             }
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder(it == 0);
@@ -406,7 +422,7 @@ struct PBFSolver::State {
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
-                        stride, col, P.particleRadius, status.data(), stream));
+                        stride, col, P.particleRadius, scenes(), status.data(), stream));
                 }
                 std::swap(p, pa);
             }
@@ -441,7 +457,7 @@ struct PBFSolver::State {
                     launch_check(pbfint::launch_confine_resolve(
                         x.data(), p.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(),
                         v.data(), vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
-                        P.particleRadius, P.collisionRestitution, P.collisionFriction,
+                        P.particleRadius, P.collisionRestitution, P.collisionFriction, scenes(),
                         status.data(), stream));
                 }
                 std::swap(v, va);
@@ -449,7 +465,7 @@ struct PBFSolver::State {
                 Scope sc(prof, "resolve_velocities", stream);
                 launch_check(pbfint::launch_resolve_guarded(
                     x.data(), vin.data(), v.data(), n, col, P.particleRadius,
-                    P.collisionRestitution, P.collisionFriction, status.data(), stream));
+                    P.collisionRestitution, P.collisionFriction, scenes(), status.data(), stream));
             }
         }
     }
@@ -562,11 +578,10 @@ PBFSolver::~PBFSolver() = default;
 PBFSolver::PBFSolver(PBFSolver&&) noexcept = default;
 PBFSolver& PBFSolver::operator=(PBFSolver&&) noexcept = default;
 
-void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
-                           const SimulationParams& params) {
-    // Validation order and exception types of pbf_solver.cu:25-79.  The
-    // neighbour-list bounds are checked like the reference's although the
-    // sweeps store no list, so the same inputs fail the same way.
+namespace {
+
+// Capacity checks of pbf_solver.cu:25-43; returns the list stride.
+std::size_t checked_stride(std::size_t maxParticles) {
     if (maxParticles == 0) throw std::invalid_argument("maxParticles must be greater than zero");
     if (maxParticles > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(kMaxNeighbors))
         throw std::length_error("Neighbor buffer size overflows size_t");
@@ -576,8 +591,11 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
         ((maxParticles + kBlockSize - 1) / kBlockSize) * static_cast<std::size_t>(kBlockSize);
     if (stride > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(kMaxNeighbors))
         throw std::length_error("Neighbor buffer size overflows size_t");
+    return stride;
+}
 
-    const SimulationParams& p = params;
+// Parameter checks of pbf_solver.cu:45-79.
+void check_params(const SimulationParams& p) {
     if (!std::isfinite(p.dt) || p.dt <= 0.0f || !std::isfinite(p.restDensity) ||
         p.restDensity <= 0.0f || !std::isfinite(p.particleMass) || p.particleMass <= 0.0f ||
         !std::isfinite(p.particleRadius) || p.particleRadius <= 0.0f ||
@@ -598,35 +616,59 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
     const float substepDt = p.dt / static_cast<float>(p.substeps);
     if (!std::isfinite(substepDt) || substepDt <= 0.0f || !std::isfinite(1.0f / substepDt))
         throw std::invalid_argument("Substep duration is not representable");
+}
 
+// The closed container must fit a particle (pbf_solver.cu:81-82).
+void check_container(float3 minBounds, float3 maxBounds, const SimulationParams& p) {
     const double width = static_cast<double>(maxBounds.x) - static_cast<double>(minBounds.x);
     const double depth = static_cast<double>(maxBounds.z) - static_cast<double>(minBounds.z);
     const double diameter = 2.0 * static_cast<double>(p.particleRadius);
     if (width < diameter || depth < diameter)
         throw std::invalid_argument("Particle diameter exceeds closed container dimensions");
+}
+
+}  // namespace
+
+// Allocates the working set, lists and tile ends for maxParticles particles
+// (after the grid is in place); the solver stays inactive until the end.
+void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles) {
+    for (DeviceBuffer<float4>* b : {&x, &xa, &p, &pa, &v, &va, &vin, &vina, &omega})
+        b->allocate(maxParticles_);
+    id.allocate(maxParticles_);
+    ida.allocate(maxParticles_);
+    status.allocate(1);
+    nbr.allocate(stride_ * static_cast<std::size_t>(kMaxNeighbors));
+    cnt.allocate(maxParticles_);
+    stride = stride_;
+    tileEnd.allocate(tiles);
+}
+
+void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
+                           const SimulationParams& params) {
+    // Validation order and exception types of pbf_solver.cu:25-79.  The
+    // neighbour-list bounds are checked like the reference's, so the same
+    // inputs fail the same way.
+    const std::size_t stride = checked_stride(maxParticles);
+    check_params(params);
+    check_container(minBounds, maxBounds, params);
 
     State& s = *state_;
-    GridCore grid = GridCore::create(maxParticles, minBounds, maxBounds, p.smoothingRadius);
+    GridCore grid = GridCore::create(maxParticles, minBounds, maxBounds, params.smoothingRadius);
     s.drop_graphs();
 
     // From here a failed allocation leaves the solver inactive
     // (pbf_solver.cu:84-87).
     s.maxParticles = 0;
     s.count = 0;
+    s.sceneCount = 0;
+    s.sceneFirst.clear();
+    s.sceneGeom = DeviceBuffer<pbfint::SceneGeom>();
+    s.sceneContainers = DeviceBuffer<pbf_container>();
+    s.sceneOf = DeviceBuffer<std::uint32_t>();
     s.grid = std::move(grid);
     s.collision.setContainer(Container{minBounds, maxBounds});
     s.ensure_stream();
-
-    for (DeviceBuffer<float4>* b : {&s.x, &s.xa, &s.p, &s.pa, &s.v, &s.va, &s.vin, &s.vina, &s.omega})
-        b->allocate(maxParticles);
-    s.id.allocate(maxParticles);
-    s.ida.allocate(maxParticles);
-    s.status.allocate(1);
-    s.nbr.allocate(stride * static_cast<std::size_t>(kMaxNeighbors));
-    s.cnt.allocate(maxParticles);
-    s.stride = stride;
-    s.tileEnd.allocate(pbfint::sweep_tile_count(s.grid.dims));
-
+    s.allocate(maxParticles, stride, pbfint::sweep_tile_count(s.grid.dims));
     s.maxParticles = maxParticles;
     s.count = 0;
     s.params = params;
@@ -788,6 +830,139 @@ void PBFSolver::copyNeighborLists(float4* positions, int* counts, std::uint32_t*
             for (std::size_t i = 0; i < n; ++i)
                 neighbors[q * n + i] = g[((q / 4) * n + i) * 4 + q % 4];
     }
+}
+
+// ---------------------------------------------------------------------------
+// PBFBatchSolver: the same step over many scenes (see pbf/pbf_batch.hpp)
+// ---------------------------------------------------------------------------
+
+PBFBatchSolver::PBFBatchSolver() = default;
+PBFBatchSolver::~PBFBatchSolver() = default;
+PBFBatchSolver::PBFBatchSolver(PBFBatchSolver&&) noexcept = default;
+PBFBatchSolver& PBFBatchSolver::operator=(PBFBatchSolver&&) noexcept = default;
+
+void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
+                                const SimulationParams& params) {
+    if (scenes.empty()) throw std::invalid_argument("A batch needs at least one scene");
+    // Capacity of every scene, then of the whole batch (one working set).
+    std::size_t total = 0;
+    for (const BatchScene& sc : scenes) {
+        (void)checked_stride(sc.particleCount);
+        if (sc.particleCount > std::numeric_limits<std::size_t>::max() - total)
+            throw std::length_error("Particle count exceeds kernel index range");
+        total += sc.particleCount;
+    }
+    const std::size_t stride = checked_stride(total);
+    if (total > std::numeric_limits<std::uint32_t>::max())
+        throw std::length_error("Particle count exceeds kernel index range");
+    check_params(params);
+
+    // Every scene's grid exactly as PBFSolver::initialize builds it (the
+    // bounds checks of spatial_grid.cu:11-27 included), laid end to end.
+    const float h = params.smoothingRadius;
+    std::vector<pbfint::SceneGeom> geom(scenes.size());
+    std::vector<pbf_container> boxes(scenes.size());
+    std::vector<std::size_t> first(scenes.size() + 1, 0);
+    std::size_t cells = 0, tiles = 0;
+    for (std::size_t k = 0; k < scenes.size(); ++k) {
+        const BatchScene& sc = scenes[k];
+        const float3 lo = sc.minBounds, hi = sc.maxBounds;
+        check_container(lo, hi, params);
+        if (!std::isfinite(lo.x) || !std::isfinite(lo.y) || !std::isfinite(lo.z) ||
+            !std::isfinite(hi.x) || !std::isfinite(hi.y) || !std::isfinite(hi.z) ||
+            hi.x <= lo.x || hi.y <= lo.y || hi.z <= lo.z)
+            throw std::invalid_argument("Invalid spatial grid bounds");
+        std::size_t sceneCells = 0;
+        const int3 d = GridCore::dimsFor(lo, hi, h, &sceneCells);
+        const std::size_t sceneTiles = pbfint::sweep_tile_count(d);
+        if (cells + sceneCells > std::numeric_limits<std::uint32_t>::max() ||
+            tiles + sceneTiles > std::numeric_limits<std::uint32_t>::max())
+            throw std::length_error("Batched grids exceed 32-bit key capacity");
+        pbfint::SceneGeom& g = geom[k];
+        g.minBounds = lo;
+        g.dims = d;
+        g.cellOff = static_cast<std::uint32_t>(cells);
+        g.tileOff = static_cast<std::uint32_t>(tiles);
+        g.chunks = static_cast<std::uint32_t>((static_cast<std::size_t>(d.x) + pbfint::kSweepTileW - 1) /
+                                              pbfint::kSweepTileW);
+        g.first = static_cast<std::uint32_t>(first[k]);
+        g.end = static_cast<std::uint32_t>(first[k] + sc.particleCount);
+        g.pad = 0;
+        boxes[k] = pbf_container{lo, hi};
+        first[k + 1] = first[k] + sc.particleCount;
+        cells += sceneCells;
+        tiles += sceneTiles;
+    }
+    std::vector<std::uint32_t> sceneOf(total);
+    for (std::size_t k = 0; k < scenes.size(); ++k)
+        for (std::size_t i = first[k]; i < first[k + 1]; ++i)
+            sceneOf[i] = static_cast<std::uint32_t>(k);
+
+    PBFSolver::State& s = *solver_.state_;
+    GridCore grid = GridCore::createKeyed(total, cells, h);
+    s.drop_graphs();
+    s.maxParticles = 0;
+    s.count = 0;
+    s.sceneCount = 0;
+    s.grid = std::move(grid);
+    s.collision.setContainer(Container{scenes[0].minBounds, scenes[0].maxBounds});
+    s.ensure_stream();
+    s.allocate(total, stride, tiles);
+    s.sceneGeom.allocate(scenes.size());
+    s.sceneGeom.copyFromHostToDevice(geom.data(), geom.size());
+    s.sceneContainers.allocate(scenes.size());
+    s.sceneContainers.copyFromHostToDevice(boxes.data(), boxes.size());
+    s.sceneOf.allocate(total);
+    s.sceneOf.copyFromHostToDevice(sceneOf.data(), total);
+    s.sceneFirst = std::move(first);
+    s.sceneCount = scenes.size();
+    s.maxParticles = total;
+    s.params = params;
+}
+
+void PBFBatchSolver::setParticles(const float4* positions, const float4* velocities,
+                                  std::size_t particleCount) {
+    const PBFSolver::State& s = *solver_.state_;
+    if (s.sceneCount == 0)
+        throw std::logic_error("PBFBatchSolver must be initialized before setting particles");
+    if (particleCount != s.maxParticles)
+        throw std::out_of_range("A batch takes every scene's particles at once");
+    solver_.setParticles(positions, velocities, particleCount);
+}
+
+void PBFBatchSolver::step() {
+    if (solver_.state_->sceneCount == 0)
+        throw std::logic_error("PBFBatchSolver must be initialized before stepping");
+    solver_.step();
+}
+
+void PBFBatchSolver::run(std::size_t frameCount) {
+    if (solver_.state_->sceneCount == 0)
+        throw std::logic_error("PBFBatchSolver must be initialized before running");
+    solver_.run(frameCount);
+}
+
+void PBFBatchSolver::copyPositionsToHost(float4* out, std::size_t particleCount) const {
+    solver_.copyPositionsToHost(out, particleCount);
+}
+
+void PBFBatchSolver::copyVelocitiesToHost(float4* out, std::size_t particleCount) const {
+    solver_.copyVelocitiesToHost(out, particleCount);
+}
+
+std::size_t PBFBatchSolver::sceneCount() const { return solver_.state_->sceneCount; }
+std::size_t PBFBatchSolver::particleCount() const { return solver_.particleCount(); }
+
+std::size_t PBFBatchSolver::sceneFirst(std::size_t scene) const {
+    const PBFSolver::State& s = *solver_.state_;
+    if (scene > s.sceneCount) throw std::out_of_range("Scene index exceeds the batch");
+    return s.sceneFirst[scene];
+}
+
+void PBFBatchSolver::setGraphs(bool enabled) { solver_.setGraphs(enabled); }
+void PBFBatchSolver::setProfiling(bool enabled) { solver_.setProfiling(enabled); }
+std::vector<PBFSolver::KernelTime> PBFBatchSolver::kernelTimes() const {
+    return solver_.kernelTimes();
 }
 
 }  // namespace pbf

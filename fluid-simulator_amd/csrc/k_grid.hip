@@ -265,14 +265,31 @@ __global__ void __launch_bounds__(kBlock) k_fill_cells(int* __restrict__ cs, int
     }
 }
 
+// Neighbour-sweep tile (kSweepTileW cells of one grid row) of key k at
+// slot i: in a batched step, the tile of the slot's scene, offset into the
+// global tile range.
+template <bool kBatched>
+__device__ __forceinline__ uint32_t sweep_tile_of(uint32_t k, size_t i, uint32_t gx,
+                                                  uint32_t chunks, const pbfint::SceneTable& st) {
+    if (kBatched) {
+        const pbfint::SceneGeom& sg = st.geom[st.sceneOf[i]];
+        const uint32_t lk = k - sg.cellOff;
+        const uint32_t sgx = static_cast<uint32_t>(sg.dims.x);
+        return sg.tileOff + (lk / sgx) * sg.chunks + (lk % sgx) / pbfint::kSweepTileW;
+    }
+    return (k / gx) * chunks + (k % gx) / pbfint::kSweepTileW;
+}
+
 // buildCellRanges (spatial_grid.cu:123-137): half-open [start, end).
 // With tileEnd not null, also records one past the last particle of every
-// non-empty neighbour-sweep tile (kSweepTileW cells of one row), which the
-// last particle of the tile writes.
+// non-empty neighbour-sweep tile, which the last particle of the tile
+// writes.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restrict__ sk,
                                                         int* __restrict__ cs, int* __restrict__ ce,
                                                         size_t n, uint32_t gx, uint32_t chunks,
-                                                        uint32_t* __restrict__ tileEnd) {
+                                                        uint32_t* __restrict__ tileEnd,
+                                                        pbfint::SceneTable st) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = sk[i];
@@ -281,8 +298,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restri
     if (!lastInCell) return;
     ce[k] = static_cast<int>(i + 1);
     if (tileEnd != nullptr) {
-        const uint32_t t = (k / gx) * chunks + (k % gx) / pbfint::kSweepTileW;
-        if (i == n - 1 || (sk[i + 1] / gx) * chunks + (sk[i + 1] % gx) / pbfint::kSweepTileW != t)
+        const uint32_t t = sweep_tile_of<kBatched>(k, i, gx, chunks, st);
+        if (i == n - 1 || sweep_tile_of<kBatched>(sk[i + 1], i + 1, gx, chunks, st) != t)
             tileEnd[t] = static_cast<uint32_t>(i + 1);
     }
 }
@@ -366,11 +383,15 @@ pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream) {
 }
 
 pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                              uint32_t* tileEnd, void* stream) {
+                              uint32_t* tileEnd, const SceneTable& st, void* stream) {
     if (n == 0) return PBF_OK;
     const uint32_t gx = static_cast<uint32_t>(dims.x);
     const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
-    PBF_LAUNCH(k_cell_ranges, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks, tileEnd);
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_cell_ranges<true>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
+                   tileEnd, st);
+    PBF_LAUNCH(k_cell_ranges<false>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
+               tileEnd, st);
 }
 
 size_t radix_scratch_words(size_t n) {

@@ -64,6 +64,25 @@ __device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ 
     }
 }
 
+// Cell key of position p: the scene's own key, offset into the global key
+// range in a batched step (the scene of slot i).
+template <bool kBatched>
+__device__ __forceinline__ uint32_t cell_key(float4 p, size_t i, pbf_float3 mn, float cs,
+                                             pbf_int3 gs, const pbfint::SceneTable& st) {
+    if (!kBatched) return hash_key(p, mn, cs, gs);
+    const pbfint::SceneGeom& sg = st.geom[st.sceneOf[i]];
+    return sg.cellOff + hash_key(p, sg.minBounds, cs, sg.dims);
+}
+
+// The colliders of slot i: in a batched step, its scene's container.
+template <bool kBatched>
+__device__ __forceinline__ Colliders colliders_of(const Colliders& col, size_t i,
+                                                  const pbfint::SceneTable& st) {
+    Colliders c = col;
+    if (kBatched) c.container = st.containers + st.sceneOf[i];
+    return c;
+}
+
 struct PosLambda {
     float4 p;
     float l;
@@ -475,11 +494,13 @@ __global__ void k_scorr(const float* __restrict__ d2, size_t n, float h, float k
 
 // integrate + collisionInputVelocities snapshot + cell key of p*
 // (pbf_solver.cu:199-208 and spatial_grid.cu:100-120 in one pass).
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_predict_key(const pbf_float4* __restrict__ x, pbf_float4* __restrict__ p,
                   pbf_float4* __restrict__ v, pbf_float4* __restrict__ vin,
                   uint32_t* __restrict__ key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                  float cs, pbf_int3 gs, const uint32_t* __restrict__ status) {
+                  float cs, pbf_int3 gs, pbfint::SceneTable st,
+                  const uint32_t* __restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (*status != 0u) return;
@@ -496,17 +517,18 @@ __global__ void __launch_bounds__(kBlock)
     st4(v, i, vi);
     st4(vin, i, vi);
     st4(p, i, pi);
-    key[i] = hash_key(pi, mn, cs, gs);
+    key[i] = cell_key<kBatched>(pi, static_cast<size_t>(i), mn, cs, gs, st);
 }
 
 // Cell keys of the current working set (computeParticleKeys,
 // spatial_grid.cu:100-120); the slot index is implicit.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_keys(const pbf_float4* __restrict__ p, uint32_t* __restrict__ key, size_t n, pbf_float3 mn,
-           float cs, pbf_int3 gs) {
+           float cs, pbf_int3 gs, pbfint::SceneTable st) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    key[i] = hash_key(ld4(p, i), mn, cs, gs);
+    key[i] = cell_key<kBatched>(ld4(p, i), i, mn, cs, gs, st);
 }
 
 // computeDeltaPosition + applyDeltaPosition + CollisionSystem::solve for one
@@ -515,12 +537,14 @@ __global__ void __launch_bounds__(kBlock)
 // positions the reference's separate Delta-p kernel sees.  pin[].w holds
 // lambda (stored there by the lambda pass); pout[].w gets back the
 // particle's own .w, which x carries.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_delta_apply_collide(const pbf_float4* __restrict__ pin, pbf_float4* __restrict__ pout,
                           const uint32_t* __restrict__ nbr, const int* __restrict__ cnt,
                           int maxN, const pbf_float4* __restrict__ x, size_t n, float h,
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
-                          Colliders col, float radius, uint32_t* __restrict__ status) {
+                          Colliders col, float radius, pbfint::SceneTable st,
+                          uint32_t* __restrict__ status) {
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -554,7 +578,7 @@ __global__ void __launch_bounds__(kBlock)
     pi.y += dpy;
     pi.z += dpz;
     pi.w = x[i].w;
-    if (!collision_fixed_point(pi, col, radius))
+    if (!collision_fixed_point(pi, colliders_of<kBatched>(col, i, st), radius))
         atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
     st4(pout, i, pi);
 }
@@ -621,6 +645,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // applyVorticityConfinement + resolveVelocitiesKernel (pbf_solver.cu:309-323).
 // pw holds the vorticity pass's (position, |omega|) records.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_confine_resolve(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ pw,
                       const uint32_t* __restrict__ nbr,
@@ -628,7 +653,7 @@ __global__ void __launch_bounds__(kBlock)
                       const pbf_float4* __restrict__ vcur, const pbf_float4* __restrict__ vsnap,
                       pbf_float4* __restrict__ vout, size_t n, float h, float dt, float strength,
                       size_t stride, Colliders col, float radius, float e, float f,
-                      const uint32_t* __restrict__ status) {
+                      pbfint::SceneTable st, const uint32_t* __restrict__ status) {
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -641,7 +666,8 @@ __global__ void __launch_bounds__(kBlock)
         const Sph s = sph_consts(h);
         confine<true>(i, pw, nbr, c, omega, vi, s, dt, strength, stride, out);
     }
-    resolve_velocity(ld4(x, i), ld4(vsnap, i), out, col, radius, e, f);
+    resolve_velocity(ld4(x, i), ld4(vsnap, i), out, colliders_of<kBatched>(col, i, st), radius, e,
+                     f);
     st4(vout, i, out);
 }
 
@@ -665,15 +691,16 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // resolveVelocitiesKernel as the last pass when confinement is off.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_resolve_guarded(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ vsnap,
                       pbf_float4* __restrict__ v, size_t n, Colliders col, float r, float e,
-                      float f, const uint32_t* __restrict__ status) {
+                      float f, pbfint::SceneTable st, const uint32_t* __restrict__ status) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (*status != 0u) return;
     float4 vi = ld4(v, i);
-    resolve_velocity(ld4(x, i), ld4(vsnap, i), vi, col, r, e, f);
+    resolve_velocity(ld4(x, i), ld4(vsnap, i), vi, colliders_of<kBatched>(col, i, st), r, e, f);
     st4(v, i, vi);
 }
 
@@ -827,9 +854,13 @@ namespace pbfint {
 
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                              float cs, pbf_int3 gs, const uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_predict_key, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g, mn, cs,
-               gs, status);
+                              float cs, pbf_int3 gs, const SceneTable& st,
+                              const uint32_t* status, void* stream) {
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_predict_key<true>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g,
+                   mn, cs, gs, st, status);
+    PBF_LAUNCH(k_predict_key<false>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g, mn,
+               cs, gs, st, status);
 }
 
 pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const Colliders& col, float radius,
@@ -843,8 +874,10 @@ pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream) {
 }
 
 pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
-                       pbf_int3 gs, void* stream) {
-    PBF_LAUNCH(k_keys, grid_for(n), kBlock, stream, p, key, n, mn, cs, gs);
+                       pbf_int3 gs, const SceneTable& st, void* stream) {
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_keys<true>, grid_for(n), kBlock, stream, p, key, n, mn, cs, gs, st);
+    PBF_LAUNCH(k_keys<false>, grid_for(n), kBlock, stream, p, key, n, mn, cs, gs, st);
 }
 
 pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
@@ -859,9 +892,12 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
                                       const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const Colliders& col, float radius,
-                                      uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_delta_apply_collide, grid_for(n), kBlock, stream, pin, pout, nbr, cnt, maxN, x,
-               n, h, mass, rho0, sk, sn, sdq, stride, col, radius, status);
+                                      const SceneTable& st, uint32_t* status, void* stream) {
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_delta_apply_collide<true>, grid_for(n), kBlock, stream, pin, pout, nbr, cnt,
+                   maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, status);
+    PBF_LAUNCH(k_delta_apply_collide<false>, grid_for(n), kBlock, stream, pin, pout, nbr, cnt,
+               maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, status);
 }
 
 pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
@@ -890,16 +926,23 @@ pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, con
                                   const pbf_float4* vcur, const pbf_float4* vsnap, pbf_float4* vout,
                                   size_t n, float h, float dt, float strength, size_t stride,
                                   const Colliders& col, float radius, float e, float f,
-                                  const uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_confine_resolve, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN, omega, vcur,
-               vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, status);
+                                  const SceneTable& st, const uint32_t* status, void* stream) {
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_confine_resolve<true>, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN,
+                   omega, vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st,
+                   status);
+    PBF_LAUNCH(k_confine_resolve<false>, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN, omega,
+               vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st, status);
 }
 
 pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,
                                   size_t n, const Colliders& col, float r, float e, float f,
-                                  const uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_resolve_guarded, grid_for(n), kBlock, stream, x, vsnap, v, n, col, r, e, f,
-               status);
+                                  const SceneTable& st, const uint32_t* status, void* stream) {
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_resolve_guarded<true>, grid_for(n), kBlock, stream, x, vsnap, v, n, col, r, e,
+                   f, st, status);
+    PBF_LAUNCH(k_resolve_guarded<false>, grid_for(n), kBlock, stream, x, vsnap, v, n, col, r, e, f,
+               st, status);
 }
 
 pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,

@@ -63,7 +63,25 @@ struct Grid {
     int chunks;  // tiles per row = ceil(dims.x / kTileW)
     float h2;
     int maxNeighbors;
+    pbfint::SceneTable st;  // a batched step's scenes
 };
+
+// The grid a tile or particle belongs to: the one grid of a single-scene
+// step, or its scene's grid in a batched step (keys and tiles offset into
+// the global ranges).
+struct Geom {
+    pbf_int3 dims;
+    uint32_t cellOff;
+    uint32_t tileOff;
+    int chunks;
+};
+
+template <bool kBatched>
+__device__ __forceinline__ Geom geom_of(const Grid& g, size_t slot) {
+    if (!kBatched) return Geom{g.dims, 0u, 0u, g.chunks};
+    const pbfint::SceneGeom& sg = g.st.geom[g.st.sceneOf[slot]];
+    return Geom{sg.dims, sg.cellOff, sg.tileOff, static_cast<int>(sg.chunks)};
+}
 
 struct Meta {
     int segFirst[kSegs];  // working-set index of the segment's first particle
@@ -102,8 +120,11 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__device__ __forceinline__ uint32_t tile_of(uint32_t key, uint32_t gx, uint32_t chunks) {
-    return (key / gx) * chunks + (key % gx) / static_cast<uint32_t>(kTileW);
+__device__ __forceinline__ uint32_t tile_of(uint32_t key, const Geom& gm) {
+    const uint32_t k = key - gm.cellOff;
+    const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
+    return gm.tileOff + (k / gx) * static_cast<uint32_t>(gm.chunks) +
+           (k % gx) / static_cast<uint32_t>(kTileW);
 }
 
 // Candidates of one row segment: staged in LDS, or read from the working
@@ -230,15 +251,17 @@ __device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const 
 // One tile: segment geometry, then the tile's particles against the three
 // z-planes of its halo.  Every branch that reaches a barrier is uniform
 // over the workgroup (it depends on LDS values written before a barrier).
-__device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, int tile) {
-    const int row = tile / g.chunks;
-    const int chunk = tile - row * g.chunks;
-    const int y = row % g.dims.y;
-    const int z = row / g.dims.y;
+__device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& m, Plane& lds,
+                                           int tile) {
+    const int local = tile - static_cast<int>(gm.tileOff);
+    const int row = local / gm.chunks;
+    const int chunk = local - row * gm.chunks;
+    const int y = row % gm.dims.y;
+    const int z = row / gm.dims.y;
     const int x0 = chunk * kTileW;
-    const int x1 = min(x0 + kTileW, g.dims.x) - 1;
+    const int x1 = min(x0 + kTileW, gm.dims.x) - 1;
     const int xs = max(x0 - 1, 0);
-    const int xe = min(x1 + 1, g.dims.x - 1);
+    const int xe = min(x1 + 1, gm.dims.x - 1);
     const int ncell = xe - xs + 1;
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63;
@@ -248,11 +271,11 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
     for (int s = wave; s < kSegs; s += kThreads / 64) {
         const int yy = y + (s % 3) - 1;
         const int zz = z + (s / 3) - 1;
-        const bool rowOk = yy >= 0 && yy < g.dims.y && zz >= 0 && zz < g.dims.z;
+        const bool rowOk = yy >= 0 && yy < gm.dims.y && zz >= 0 && zz < gm.dims.z;
         int cnt = 0;
         int st = 0x7fffffff;
         if (rowOk && lane < ncell) {
-            const uint32_t key = key_of(xs + lane, yy, zz, g.dims);
+            const uint32_t key = gm.cellOff + key_of(xs + lane, yy, zz, gm.dims);
             const int a = g.cellStart[key];
             if (a != -1) {
                 cnt = g.cellEnd[key] - a;
@@ -301,7 +324,8 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
         if (active) {
             const float4 p = ld4(g.p, gi);
             ip = make_float3(p.x, p.y, p.z);
-            mc = static_cast<int>(g.sortedKeys[gi] % static_cast<uint32_t>(g.dims.x)) - xs;
+            mc = static_cast<int>((g.sortedKeys[gi] - gm.cellOff) %
+                                  static_cast<uint32_t>(gm.dims.x)) - xs;
         }
         for (int pl = 0; pl < 3; ++pl) {
             const int pc = m.planeCount[pl];
@@ -352,25 +376,27 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, Meta& m, Plane& lds, i
 // One particle on one lane, candidates read from global memory: the row
 // range of every (dz, dy) is the run of particles from the first non-empty
 // cell of cx-1 .. cx+1 to the last, as in the tile form.
+template <bool kBatched>
 __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
-    const uint32_t key = g.sortedKeys[gi];
-    const uint32_t gx = static_cast<uint32_t>(g.dims.x);
+    const Geom gm = geom_of<kBatched>(g, gi);
+    const uint32_t key = g.sortedKeys[gi] - gm.cellOff;
+    const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
     const int cx = static_cast<int>(key % gx);
     const uint32_t rowId = key / gx;
-    const int cy = static_cast<int>(rowId % static_cast<uint32_t>(g.dims.y));
-    const int cz = static_cast<int>(rowId / static_cast<uint32_t>(g.dims.y));
+    const int cy = static_cast<int>(rowId % static_cast<uint32_t>(gm.dims.y));
+    const int cz = static_cast<int>(rowId / static_cast<uint32_t>(gm.dims.y));
     const float4 p = ld4(g.p, gi);
     const float3 ip = make_float3(p.x, p.y, p.z);
     ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
     const int xa = max(cx - 1, 0);
-    const int xb = min(cx + 1, g.dims.x - 1);
+    const int xb = min(cx + 1, gm.dims.x - 1);
     for (int s = 0; s < kSegs; ++s) {
         const int yy = cy + (s % 3) - 1;
         const int zz = cz + (s / 3) - 1;
-        if (yy < 0 || yy >= g.dims.y || zz < 0 || zz >= g.dims.z) continue;
+        if (yy < 0 || yy >= gm.dims.y || zz < 0 || zz >= gm.dims.z) continue;
         int rowStart = -1, rowEnd = -1;
         for (int x = xa; x <= xb; ++x) {
-            const uint32_t k = key_of(x, yy, zz, g.dims);
+            const uint32_t k = gm.cellOff + key_of(x, yy, zz, gm.dims);
             const int a = g.cellStart[k];
             if (a == -1) continue;
             if (rowStart < 0) rowStart = a;
@@ -390,6 +416,7 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
 // waves per SIMD (72 VGPRs, three spilled on cold paths) rather than the six
 // that 77 VGPRs allow: the sweep waits on LDS and the list stores most of
 // the time, and the seventh wave hides part of it.
+template <bool kBatched>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_build_lists(Grid g) {
     __shared__ Meta m;
@@ -401,13 +428,11 @@ k_build_lists(Grid g) {
     const size_t i = static_cast<size_t>(xcd_task(static_cast<int>(blockIdx.x),
                                                   static_cast<int>(gridDim.x))) *
                          kThreads + static_cast<size_t>(tid);
-    const uint32_t gx = static_cast<uint32_t>(g.dims.x);
-    const uint32_t chunks = static_cast<uint32_t>(g.chunks);
     bool starts = false;
     uint32_t t = 0;
     if (i < g.n) {
-        t = tile_of(g.sortedKeys[i], gx, chunks);
-        starts = i == 0 || tile_of(g.sortedKeys[i - 1], gx, chunks) != t;
+        t = tile_of(g.sortedKeys[i], geom_of<kBatched>(g, i));
+        starts = i == 0 || tile_of(g.sortedKeys[i - 1], geom_of<kBatched>(g, i - 1)) != t;
     }
     // Tiles in particle order: wave counts, then ranks within the wave.
     const uint64_t ballot = __ballot(starts);
@@ -431,13 +456,15 @@ k_build_lists(Grid g) {
     }
     lds_barrier();
     if (total <= kMaxTilePasses) {
-        for (int k = 0; k < total; ++k) sweep_tile(g, m, lds, m.tiles[k]);
+        for (int k = 0; k < total; ++k)
+            sweep_tile(g, geom_of<kBatched>(g, static_cast<size_t>(m.tileFirst[k])), m, lds,
+                       m.tiles[k]);
     } else {
         // The particles of every tile that starts here, one lane each.
         const size_t first = static_cast<size_t>(m.tileFirst[0]);
         const size_t end = g.tileEnd[m.tiles[total - 1]];
         for (size_t q = first + static_cast<size_t>(tid); q < end; q += kThreads)
-            sweep_lane(g, q);
+            sweep_lane<kBatched>(g, q);
     }
 }
 
@@ -474,13 +501,16 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
     g.chunks = (sg.dims.x + kTileW - 1) / kTileW;
     g.h2 = sg.h * sg.h;
     g.maxNeighbors = sg.maxNeighbors;
-    const size_t tiles = sweep_tile_count(sg.dims);
+    g.st = sg.st;
+    const size_t tiles = sg.st.geom != nullptr ? 0 : sweep_tile_count(sg.dims);
     if (tiles > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "grid has too many rows for the neighbour sweep");
     const size_t tasks = (sg.n + kThreads - 1) / kThreads;
     if (tasks > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "too many particles for the neighbour sweep");
-    PBF_LAUNCH(k_build_lists, static_cast<unsigned>(tasks), kThreads, stream, g);
+    if (sg.st.geom != nullptr)
+        PBF_LAUNCH(k_build_lists<true>, static_cast<unsigned>(tasks), kThreads, stream, g);
+    PBF_LAUNCH(k_build_lists<false>, static_cast<unsigned>(tasks), kThreads, stream, g);
 }
 
 }  // namespace pbfint

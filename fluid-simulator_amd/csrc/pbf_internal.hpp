@@ -10,6 +10,28 @@
 
 namespace pbfint {
 
+// Scenes of a batched step.  Scene s keeps its own grid (bounds, dims) and
+// container; its particles occupy the slots [first, end) in every order the
+// step produces (cell order sorts by global key = cellOff + the scene's own
+// key, so scenes stay in their slot ranges), and its sweep tiles are the
+// global tiles tileOff + (the scene's own tile index).
+struct SceneGeom {
+    pbf_float3 minBounds;
+    pbf_int3 dims;
+    uint32_t cellOff;
+    uint32_t tileOff;
+    uint32_t chunks;  // sweep tiles per grid row
+    uint32_t first;
+    uint32_t end;
+    uint32_t pad;
+};
+// Device tables of a batched step; all null for a single scene.
+struct SceneTable {
+    const SceneGeom* geom = nullptr;
+    const pbf_container* containers = nullptr;
+    const uint32_t* sceneOf = nullptr;  // scene of every slot
+};
+
 // Grid (k_grid.hip)
 long long stencil_radius(float h, float cellSize);
 pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* ce, pbf_int3 g,
@@ -24,7 +46,7 @@ pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
 // every non-empty sweep tile (sweep_tile_count(dims) entries, only those of
 // non-empty tiles written).
 pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                              uint32_t* tileEnd, void* stream);
+                              uint32_t* tileEnd, const SceneTable& st, void* stream);
 size_t radix_scratch_words(size_t n);
 int radix_key_bits(size_t numCells);
 pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
@@ -34,9 +56,10 @@ pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* 
 // Solver (k_solver.hip)
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                              float cs, pbf_int3 gs, const uint32_t* status, void* stream);
+                              float cs, pbf_int3 gs, const SceneTable& st,
+                              const uint32_t* status, void* stream);
 pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
-                       pbf_int3 gs, void* stream);
+                       pbf_int3 gs, const SceneTable& st, void* stream);
 // lambda goes to p[].w (the step's Delta-p reads it there).
 pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
                                  size_t n, float h, float mass, float rho0, float eps,
@@ -46,7 +69,7 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
                                       const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const pbfdev::Colliders& col, float radius,
-                                      uint32_t* status, void* stream);
+                                      const SceneTable& st, uint32_t* status, void* stream);
 pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
                               pbf_float4* vout, const uint32_t* nbr, const int* cnt, int maxN,
                               size_t n, float h, float visc, float inv_dt, size_t stride,
@@ -64,10 +87,11 @@ pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, con
                                   const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
                                   float dt, float strength, size_t stride,
                                   const pbfdev::Colliders& col, float radius, float e, float f,
-                                  const uint32_t* status, void* stream);
+                                  const SceneTable& st, const uint32_t* status, void* stream);
 pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,
                                   size_t n, const pbfdev::Colliders& col, float r, float e,
-                                  float f, const uint32_t* status, void* stream);
+                                  float f, const SceneTable& st, const uint32_t* status,
+                                  void* stream);
 pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
                          const pbf_float4* v, const pbf_float4* vin, const uint32_t* id,
                          pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
@@ -82,10 +106,11 @@ struct SweepGrid {
     const int* cellEnd;
     const uint32_t* sortedKeys;
     const uint32_t* tileEnd;
-    pbf_int3 dims;
+    pbf_int3 dims;  // one scene's grid (unused when st is set)
     size_t n;
     float h;
     int maxNeighbors;
+    SceneTable st;  // a batched step's scenes
 };
 // Entries of the tileEnd array for a grid of these dims.
 size_t sweep_tile_count(pbf_int3 dims);

@@ -26,6 +26,7 @@ import numpy as np
 __all__ = [
     "lib", "LIB_PATH", "float3", "int3", "SimulationParams", "Container", "SphereCollider",
     "BoxCollider", "PlaneCollider", "make_float3", "DeviceArray", "PBFSolver", "SpatialGrid",
+    "PBFBatchSolver", "pinned_empty",
     "CollisionSystem", "kernels", "PbfError", "InvalidArgument", "LogicError", "OutOfRange",
     "LengthError", "NeighborOverflow", "CollisionNonConvergence", "DeviceError",
     "STATUS_NEIGHBOR_OVERFLOW", "STATUS_COLLISION_NONCONVERGENCE",
@@ -248,6 +249,19 @@ _SIGNATURES = {
     "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
     "pbf_solver_neighbor_lists": (_st, [_vp, _vp, _vp, _vp, _sz, _i]),
+    "pbf_batch_create": (_st, [C.POINTER(_vp)]),
+    "pbf_batch_destroy": (_st, [_vp]),
+    "pbf_batch_initialize": (_st, [_vp, _vp, _sz, _vp]),
+    "pbf_batch_set_particles": (_st, [_vp, _vp, _vp, _sz]),
+    "pbf_batch_step": (_st, [_vp]),
+    "pbf_batch_run": (_st, [_vp, _sz]),
+    "pbf_batch_copy_positions_to_host": (_st, [_vp, _vp, _sz]),
+    "pbf_batch_copy_velocities_to_host": (_st, [_vp, _vp, _sz]),
+    "pbf_batch_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
+    "pbf_batch_set_graphs": (_st, [_vp, _i]),
+    "pbf_batch_set_profiling": (_st, [_vp, _i]),
+    "pbf_batch_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
 }
 
 _LIB = None
@@ -646,3 +660,79 @@ class PBFSolver:
 
 
 from . import kernels  # noqa: E402  (device kernel wrappers need the classes above)
+
+
+class _batch_scene(C.Structure):
+    _fields_ = [("particleCount", C.c_size_t), ("minBounds", float3), ("maxBounds", float3)]
+
+
+class PBFBatchSolver:
+    """Many independent scenes advanced together (pbf_batch_*): each scene
+    keeps its own grid and container, the parameters are shared, and every
+    stage of the step is one launch for all scenes.  Each scene's result is
+    bitwise that of a PBFSolver holding it alone."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().pbf_batch_create(C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pbf_batch_destroy(self._h)
+            self._h = None
+
+    def initialize(self, scenes, params: SimulationParams):
+        """scenes: (particleCount, minBounds, maxBounds) triples."""
+        arr = (_batch_scene * max(len(scenes), 1))()
+        for k, (n, lo, hi) in enumerate(scenes):
+            arr[k] = _batch_scene(int(n), _f3(lo), _f3(hi))
+        _check(lib().pbf_batch_initialize(self._h, C.cast(arr, _vp), len(scenes),
+                                          C.cast(C.byref(params), _vp)))
+
+    def setParticles(self, positions, velocities, particleCount=None):
+        pos = None if positions is None else as_float4(positions)
+        vel = None if velocities is None else as_float4(velocities)
+        n = particleCount if particleCount is not None else (0 if pos is None else len(pos))
+        _check(lib().pbf_batch_set_particles(self._h, _ptr(pos), _ptr(vel), int(n)))
+
+    def step(self):
+        _check(lib().pbf_batch_step(self._h))
+
+    def run(self, frames: int):
+        _check(lib().pbf_batch_run(self._h, int(frames)))
+
+    def counts(self):
+        s, n = _sz(), _sz()
+        _check(lib().pbf_batch_counts(self._h, C.byref(s), C.byref(n)))
+        return s.value, n.value
+
+    def _copy(self, fn, out):
+        n = self.counts()[1]
+        out = np.zeros((n, 4), dtype=np.float32) if out is None else out
+        _check(fn(self._h, _ptr(out), n))
+        return out
+
+    def copyPositionsToHost(self, out=None):
+        return self._copy(lib().pbf_batch_copy_positions_to_host, out)
+
+    def copyVelocitiesToHost(self, out=None):
+        return self._copy(lib().pbf_batch_copy_velocities_to_host, out)
+
+    def setGraphs(self, enabled: bool):
+        _check(lib().pbf_batch_set_graphs(self._h, 1 if enabled else 0))
+
+    def setProfiling(self, enabled: bool):
+        _check(lib().pbf_batch_set_profiling(self._h, 1 if enabled else 0))
+
+    def kernelTimes(self) -> dict:
+        cap = 64
+        names = C.create_string_buffer(8192)
+        ms = (C.c_double * cap)()
+        launches = (C.c_uint64 * cap)()
+        count = _sz()
+        _check(lib().pbf_batch_kernel_times(self._h, names, 8192, ms, launches, cap,
+                                            C.byref(count)))
+        keys = names.value.decode().split("\n")[:count.value]
+        return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
+

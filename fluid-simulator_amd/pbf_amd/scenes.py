@@ -166,3 +166,34 @@ def small_block(nx: int, ny: int, nz: int, seed: int = 0x5EED00FF, box_scale=(4,
     vel = np.zeros_like(pos)
     return Scene(f"block_{nx}x{ny}x{nz}", pos, vel, (0.0, 0.0, 0.0),
                  tuple(float(np.float32(v)) for v in box), params)
+
+
+def batch_scene(index: int, nmin: int = 2048, nmax: int = 200_000, substeps: int = 2,
+                iterations: int = 4) -> Scene:
+    """Scene `index` of the batched workload: a dam-break block of
+    n = round(exp(U(ln nmin, ln nmax))) particles with aspect ratios
+    a1, a2 ~ U[0.5, 2] (nz = round(cbrt(n / (a1 a2))), nx = round(a1 nz),
+    ny = ceil(n / (nx nz)), the fill truncated to n), in its own open-top box
+    of the dam-break construction.  Seed 0xBA7C0000 + index: its first three
+    SplitMix64 outputs draw n, a1 and a2, the rest the jitter.  Two substeps
+    per frame, as the 1M dam break (the tallest blocks are about 3 m)."""
+    seed = 0xBA7C0000 + index
+    u = (splitmix64(seed, 3) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    n = int(round(np.exp(np.log(nmin) + u[0] * (np.log(nmax) - np.log(nmin)))))
+    a1, a2 = 0.5 + 1.5 * u[1], 0.5 + 1.5 * u[2]
+    nz = max(1, int(round(np.cbrt(n / (a1 * a2)))))
+    nx = max(1, int(round(a1 * nz)))
+    ny = -(-n // (nx * nz))
+    pos = lattice_block(nx, ny, nz, seed, first=1)[:n]
+    box = (4 * nx * SPACING, 2 * ny * SPACING, nz * SPACING)
+    vel = np.zeros_like(pos)
+    params = dam_break_params(substeps, iterations)
+    return Scene(f"batch_{index}", pos, vel, (0.0, 0.0, 0.0),
+                 tuple(float(np.float32(v)) for v in box), params)
+
+
+def batch_scenes(count: int = 256, **kw) -> list:
+    """The batched workload: scenes 0 .. count-1 (about 43,000 particles
+    each on average, 11.06M for 256)."""
+    return [batch_scene(k, **kw) for k in range(count)]
+

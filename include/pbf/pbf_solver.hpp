@@ -74,6 +74,7 @@ public:
     struct State;  // library-internal
 
 private:
+    friend class PBFBatchSolver;  // runs the same step over a batch of scenes
     std::unique_ptr<State> state_;
 };
 

@@ -390,6 +390,42 @@ pbf_status pbf_solver_neighbor_lists(pbf_solver* solver, pbf_float4* h_positions
                                      uint32_t* h_neighbors, size_t particleCount,
                                      int maxNeighbors);
 
+/* ------------------------------------------------------------------------ */
+/* Batched scenes (PBFBatchSolver, include/pbf/pbf_batch.hpp).  No          */
+/* counterpart in the reference, whose callers loop PBFSolver over scenes:  */
+/* many independent scenes, each with its own grid and container (its      */
+/* bounds) and shared parameters, advanced by one launch per stage.  Each   */
+/* scene's result equals, bit for bit, a pbf_solver holding it alone.       */
+/* ------------------------------------------------------------------------ */
+typedef struct pbf_batch pbf_batch;
+typedef struct pbf_batch_scene {
+    size_t particleCount;
+    pbf_float3 minBounds;
+    pbf_float3 maxBounds;
+} pbf_batch_scene;
+
+pbf_status pbf_batch_create(pbf_batch** batch);
+pbf_status pbf_batch_destroy(pbf_batch* batch);
+/* Validates every scene as pbf_solver_initialize would and allocates. */
+pbf_status pbf_batch_initialize(pbf_batch* batch, const pbf_batch_scene* scenes,
+                                size_t sceneCount, const pbf_sim_params* params);
+/* All particles, scene after scene; particleCount = the scenes' total. */
+pbf_status pbf_batch_set_particles(pbf_batch* batch, const pbf_float4* h_positions,
+                                   const pbf_float4* h_velocities, size_t particleCount);
+pbf_status pbf_batch_step(pbf_batch* batch);
+pbf_status pbf_batch_run(pbf_batch* batch, size_t frameCount);
+/* All particles, in the order of pbf_batch_set_particles. */
+pbf_status pbf_batch_copy_positions_to_host(pbf_batch* batch, pbf_float4* h_out,
+                                            size_t particleCount);
+pbf_status pbf_batch_copy_velocities_to_host(pbf_batch* batch, pbf_float4* h_out,
+                                             size_t particleCount);
+pbf_status pbf_batch_counts(const pbf_batch* batch, size_t* sceneCount, size_t* particleCount);
+pbf_status pbf_batch_set_graphs(pbf_batch* batch, int enabled);
+pbf_status pbf_batch_set_profiling(pbf_batch* batch, int enabled);
+pbf_status pbf_batch_kernel_times(pbf_batch* batch, char* names, size_t namesBytes,
+                                  double* milliseconds, uint64_t* launches, size_t capacity,
+                                  size_t* kernelCount);
+
 #ifdef __cplusplus
 }
 #endif

@@ -406,3 +406,26 @@ def test_upload_download_and_smaller_replacement(gpu):
     x2, v2 = f4([3, 4, 5, 6], [7, 8, 9, 10]), f4([-1, -2, -3, 11], [-4, -5, -6, 12])
     s.setParticles(x2, v2)
     assert np.array_equal(s.copyPositionsToHost(), x2) and np.array_equal(s.copyVelocitiesToHost(), v2)
+
+
+def test_batch_validation():
+    """The batch validates like PBFSolver::initialize, per scene, before it
+    touches the device."""
+    import pbf_amd
+
+    b = pbf_amd.PBFBatchSolver()
+    p = valid_params()
+    with pytest.raises(pbf_amd.InvalidArgument):
+        b.initialize([], p)
+    with pytest.raises(pbf_amd.InvalidArgument):
+        b.initialize([(10, MIN, MAX), (0, MIN, MAX)], p)
+    with pytest.raises(pbf_amd.InvalidArgument):
+        b.initialize([(10, MIN, MAX), (10, (0, 0, 0), (1, -1, 1))], p)
+    with pytest.raises(pbf_amd.InvalidArgument):
+        b.initialize([(10, MIN, MAX)], valid_params(dt=-1.0))
+    with pytest.raises(pbf_amd.InvalidArgument):
+        b.initialize([(10, MIN, MAX), (10, (0, 0, 0), (0.01, 1, 1))], p)
+    with pytest.raises(pbf_amd.LogicError):
+        b.setParticles(np.zeros((1, 4), np.float32), np.zeros((1, 4), np.float32))
+    with pytest.raises(pbf_amd.LogicError):
+        b.step()
