@@ -22,7 +22,14 @@ Prints one JSON line:
   valu           the same kernel against the unpacked FP32 vector peak
                  (flop model per neighbour and per candidate test)
   step_roofline  the whole step's algorithmic bytes per particle-step x
-                 value, against 8 TB/s
+                 value, against 8 TB/s (this dataflow: five grid builds and
+                 five list builds per substep, as the reference)
+  b_alg_roofline the same with the compulsory-byte model of one build per
+                 step and no stored lists (356 + 56 I + 16 ceil(b/8) + 8 C/N)
+  batched        the batched workload (256 seeded scenes of 2,048-200,000
+                 particles, about 10M in all): one PBFBatchSolver against a
+                 loop of PBFSolver.step() over one solver per scene, and the
+                 speedup (--no-batched skips it)
   cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on
                  the host cores, on a bounded sample of the same workload
 
@@ -172,6 +179,46 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool):
     return elapsed, times
 
 
+def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2) -> dict:
+    """The batched workload (`count` seeded scenes of 2,048-200,000
+    particles, shared parameters): one PBFBatchSolver against looping
+    PBFSolver.step() over one solver per scene, each frame.  Every scene's
+    result is bitwise the same either way (tests/test_batch_gpu.py)."""
+    import numpy as np
+
+    scs = scenes.batch_scenes(count)
+    n = sum(s.n for s in scs)
+    sub = scs[0].params.substeps
+    b = pbf_amd.PBFBatchSolver()
+    b.initialize([(s.n, s.minBounds, s.maxBounds) for s in scs], scs[0].params)
+    b.setParticles(np.concatenate([s.positions for s in scs]),
+                   np.concatenate([s.velocities for s in scs]))
+    b.run(warmup)
+    pbf_amd.synchronize()
+    t0 = time.perf_counter()
+    b.run(frames)
+    pbf_amd.synchronize()
+    tb = (time.perf_counter() - t0) / frames
+    del b
+    solvers = []
+    for s in scs:
+        o = pbf_amd.PBFSolver()
+        o.initialize(s.n, s.minBounds, s.maxBounds, s.params)
+        o.setParticles(s.positions, s.velocities)
+        o.run(warmup)
+        solvers.append(o)
+    pbf_amd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        for o in solvers:
+            o.step()
+    tl = (time.perf_counter() - t0) / frames
+    return {"scenes": count, "particles": n, "substeps": sub, "frames": frames,
+            "batched_value": n * sub / tb, "looped_value": n * sub / tl,
+            "unit": "particle-steps/s", "batched_ms_per_step": tb * 1e3,
+            "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +227,9 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-pass", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-versus-looped measurement (256 scenes)")
+    ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
     a = ap.parse_args()
 
@@ -246,6 +296,16 @@ def main() -> int:
                  "achieved": bstep * value / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": bstep * value / 1e9 / HBM_PEAK_GBS}
 
+    # The survey's algorithmic byte model of a step (one grid build, no
+    # materialised lists): B_alg = 356 + 56 I + 16 ceil(b / 8) + 8 C / N.
+    b_alg = (356 + 56 * scene.params.solverIterations + 16 * -(-key_bits // 8) + 8 * cells / n)
+    b_alg_roof = {"bytes_per_particle_step": b_alg, "achieved": b_alg * value / 1e9,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_alg * value / 1e9 / HBM_PEAK_GBS}
+
+    batched = None
+    if not a.no_batched:
+        batched = batched_vs_looped(pbf_amd, scenes, a.batch_scenes, frames=3)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene)
@@ -274,6 +334,8 @@ def main() -> int:
         "roofline": roofline,
         "valu": valu,
         "step_roofline": step_roof,
+        "b_alg_roofline": b_alg_roof,
+        "batched": batched,
         "cpu_baseline": cpu,
         "kernels": kernel_rows,
     }
