@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "host_internal.hpp"
+#include "pbf/pbf_adjoint.hpp"
 #include "pbf/pbf_batch.hpp"
 #include "pbf/pbf_solver.hpp"
 #include "pbf_amd.h"
@@ -21,6 +22,10 @@ struct pbf_solver {
 };
 struct pbf_batch {
     pbf::PBFBatchSolver batch;
+};
+struct pbf_adjoint {
+    explicit pbf_adjoint(pbf::PBFSolver& s) : adjoint(s) {}
+    pbf::PBFAdjoint adjoint;
 };
 
 namespace {
@@ -374,6 +379,29 @@ pbf_status pbf_batch_kernel_times(pbf_batch* b, char* names, size_t namesBytes, 
         write_kernel_times(b->batch.kernelTimes(), names, namesBytes, ms, launches, capacity,
                            kernelCount);
     });
+}
+
+pbf_status pbf_adjoint_create(pbf_solver* s, pbf_adjoint** adjoint) {
+    PBF_REQUIRE(s);
+    if (adjoint == nullptr) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "null output pointer");
+    return guarded([&] { *adjoint = new pbf_adjoint(s->solver); });
+}
+
+pbf_status pbf_adjoint_destroy(pbf_adjoint* adjoint) {
+    delete adjoint;
+    return PBF_OK;
+}
+
+pbf_status pbf_adjoint_forward(pbf_adjoint* a, size_t frames, size_t every) {
+    PBF_REQUIRE(a);
+    return guarded([&] { a->adjoint.forward(frames, every); });
+}
+
+pbf_status pbf_adjoint_backward(pbf_adjoint* a, const pbf_float4* xb, const pbf_float4* vb,
+                                size_t n, pbf_float4* x0b, pbf_float4* v0b, double* cb,
+                                double* eb) {
+    PBF_REQUIRE(a);
+    return guarded([&] { a->adjoint.backward(xb, vb, n, x0b, v0b, cb, eb); });
 }
 
 }  // extern "C"

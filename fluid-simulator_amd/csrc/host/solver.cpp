@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "host_internal.hpp"
+#include "pbf/pbf_adjoint.hpp"
 #include "pbf/pbf_batch.hpp"
 #include "pbf/pbf_solver.hpp"
 #include "pbf_internal.hpp"
@@ -202,6 +203,33 @@ bool device_visible(const void* p) {
 
 }  // namespace
 
+// What the adjoint needs of one substep of the forward, in the working-set
+// order of each grid build (the four constraint iterations, then the
+// post-solve rebuild): the particle id of every slot, the grouped lists
+// and counts, the positions entering and leaving each iteration (lambda in
+// .w of the former), and the post-solve phase's records.
+struct SubstepTape {
+    DeviceBuffer<std::uint32_t> ids[5];
+    DeviceBuffer<std::uint32_t> nbr[5];
+    DeviceBuffer<int> cnt[5];
+    DeviceBuffer<float4> pk[4], pout[4];
+    DeviceBuffer<float4> xs, pf, v3, omega, vin;
+
+    void allocate(std::size_t n, std::size_t listWords) {
+        if (xs.size() >= n && nbr[0].size() >= listWords) return;
+        for (int b = 0; b < 5; ++b) {
+            ids[b].allocate(n);
+            nbr[b].allocate(listWords);
+            cnt[b].allocate(n);
+        }
+        for (int k = 0; k < 4; ++k) {
+            pk[k].allocate(n);
+            pout[k].allocate(n);
+        }
+        for (DeviceBuffer<float4>* b : {&xs, &pf, &v3, &omega, &vin}) b->allocate(n);
+    }
+};
+
 struct PBFSolver::State {
     std::size_t maxParticles = 0;
     std::size_t count = 0;
@@ -251,6 +279,25 @@ struct PBFSolver::State {
     std::vector<FrameGraph> graphs;
     bool useGraphs = false;
 
+    // Set while the adjoint replays a checkpoint segment: every substep of
+    // the frames records into tapes[tapeNext++].
+    std::vector<SubstepTape>* tapes = nullptr;
+    std::size_t tapeNext = 0;
+
+    template <typename T>
+    void tape_copy(DeviceBuffer<T>& dst, const DeviceBuffer<T>& src, std::size_t count) {
+        hip_check(hipMemcpyAsync(dst.data(), src.data(), count * sizeof(T),
+                                 hipMemcpyDeviceToDevice, stream));
+    }
+    SubstepTape* tape() { return tapes != nullptr ? &(*tapes)[tapeNext] : nullptr; }
+    void tape_build(int b) {
+        SubstepTape* tp = tape();
+        if (tp == nullptr) return;
+        tape_copy(tp->ids[b], id, count);
+        tape_copy(tp->nbr[b], nbr, nbr.size());
+        tape_copy(tp->cnt[b], cnt, count);
+    }
+
     ~State() {
         drop_graphs();
         if (stream != nullptr) (void)hipStreamDestroy(stream);
@@ -295,7 +342,7 @@ struct PBFSolver::State {
     // assignment, captured now if there is none; launched eagerly when
     // graphs are off or the step is being profiled (events per launch).
     void frame() {
-        if (!useGraphs || prof.enabled) {
+        if (!useGraphs || prof.enabled || tapes != nullptr) {
             enqueue_frame();
             return;
         }
@@ -411,12 +458,14 @@ struct PBFSolver::State {
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder(it == 0);
                 build_lists();
+                tape_build(it);
                 {
                     Scope sc(prof, "lambda", stream);
                     launch_check(pbfint::launch_lambda_guarded(
                         p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
                         P.restDensity, P.lambdaEpsilon, stride, status.data(), stream));
                 }
+                if (SubstepTape* tp = tape()) tape_copy(tp->pk[it], p, n);
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
                     launch_check(pbfint::launch_delta_apply_collide(
@@ -424,12 +473,19 @@ struct PBFSolver::State {
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
                         stride, col, P.particleRadius, scenes(), status.data(), stream));
                 }
+                if (SubstepTape* tp = tape()) tape_copy(tp->pout[it], pa, n);
                 std::swap(p, pa);
             }
             const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
             if (post) {
                 build_and_reorder(false);
                 build_lists();
+                tape_build(4);
+                if (SubstepTape* tp = tape()) {
+                    tape_copy(tp->xs, x, n);
+                    tape_copy(tp->pf, p, n);
+                    tape_copy(tp->vin, vin, n);
+                }
             }
             if (P.xsphViscosity != 0.0f) {
                 Scope sc(prof, "update_xsph", stream);
@@ -442,6 +498,7 @@ struct PBFSolver::State {
                 launch_check(pbfint::launch_update_velocity_only(x.data(), p.data(), v.data(), n,
                                                                  invDt, status.data(), stream));
             }
+            if (SubstepTape* tp = tape()) tape_copy(tp->v3, v, n);
             std::swap(x, p);
             if (P.vorticityStrength != 0.0f) {
                 {
@@ -452,6 +509,7 @@ struct PBFSolver::State {
                                                           cnt.data(), kMaxNeighbors, n, h,
                                                           omega.data(), p.data(), stride, stream));
                 }
+                if (SubstepTape* tp = tape()) tape_copy(tp->omega, omega, n);
                 {
                     Scope sc(prof, "confine_resolve", stream);
                     launch_check(pbfint::launch_confine_resolve(
@@ -467,6 +525,7 @@ struct PBFSolver::State {
                     x.data(), vin.data(), v.data(), n, col, P.particleRadius,
                     P.collisionRestitution, P.collisionFriction, scenes(), status.data(), stream));
             }
+            if (tapes != nullptr) ++tapeNext;
         }
     }
 
@@ -963,6 +1022,186 @@ void PBFBatchSolver::setGraphs(bool enabled) { solver_.setGraphs(enabled); }
 void PBFBatchSolver::setProfiling(bool enabled) { solver_.setProfiling(enabled); }
 std::vector<PBFSolver::KernelTime> PBFBatchSolver::kernelTimes() const {
     return solver_.kernelTimes();
+}
+
+// ---------------------------------------------------------------------------
+// PBFAdjoint: checkpointed forward, taped replay, adjoint kernels
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// The working set at a frame boundary (slot order, with the slot ids).
+struct Checkpoint {
+    DeviceBuffer<float4> x, v;
+    DeviceBuffer<std::uint32_t> id;
+};
+
+}  // namespace
+
+struct PBFAdjoint::Impl {
+    PBFSolver* solver = nullptr;
+    std::size_t frames = 0;
+    std::size_t every = 0;
+    std::size_t n = 0;
+    std::vector<Checkpoint> checkpoints;  // frame 0, every, 2 every, ...
+    Checkpoint last;                      // the state forward() ended in
+    std::vector<SubstepTape> tapes;
+    DeviceBuffer<float4> pb, vb, x0b, v1b, v3b, v2b, etab, wb, prb, gsum;
+    DeviceBuffer<float> wlb, cterm, lamb;
+    DeviceBuffer<double> acc;
+
+    PBFSolver::State& S() { return *solver->state_; }
+
+    void save(Checkpoint& c) {
+        PBFSolver::State& s = S();
+        c.x.allocate(n);
+        c.v.allocate(n);
+        c.id.allocate(n);
+        s.tape_copy(c.x, s.x, n);
+        s.tape_copy(c.v, s.v, n);
+        s.tape_copy(c.id, s.id, n);
+    }
+    void restore(const Checkpoint& c) {
+        PBFSolver::State& s = S();
+        const auto copy = [&](auto& dst, const auto& src, std::size_t bytes) {
+            hip_check(hipMemcpyAsync(dst.data(), src.data(), bytes, hipMemcpyDeviceToDevice,
+                                     s.stream));
+        };
+        copy(s.x, c.x, n * sizeof(float4));
+        copy(s.v, c.v, n * sizeof(float4));
+        copy(s.id, c.id, n * sizeof(std::uint32_t));
+    }
+};
+
+PBFAdjoint::PBFAdjoint(PBFSolver& solver) : impl_(new Impl()) { impl_->solver = &solver; }
+PBFAdjoint::~PBFAdjoint() = default;
+
+void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
+    Impl& m = *impl_;
+    PBFSolver::State& s = m.S();
+    if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before running");
+    if (s.sceneCount != 0) throw std::invalid_argument("The adjoint takes a single-scene solver");
+    if (checkpointEvery == 0) throw std::invalid_argument("checkpointEvery must be positive");
+    const pbfdev::Colliders col = s.collision.state().view();
+    if (col.sphereCount != 0 || col.boxCount != 0 || col.planeCount != 0)
+        throw std::invalid_argument("The adjoint models the container only");
+    if (!(s.params.vorticityStrength > 0.0f))
+        throw std::invalid_argument("The adjoint needs vorticity confinement enabled");
+    m.frames = frameCount;
+    m.every = checkpointEvery;
+    m.n = s.count;
+    m.checkpoints.clear();
+    m.tapes.clear();
+    s.reset_status();
+    for (std::size_t f = 0; f < frameCount; ++f) {
+        if (f % checkpointEvery == 0) {
+            m.checkpoints.emplace_back();
+            m.save(m.checkpoints.back());
+        }
+        s.frame();
+    }
+    m.save(m.last);
+    s.finish();
+}
+
+void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t particleCount,
+                          float4* x0Bar, float4* v0Bar, double* viscosityBar,
+                          double* vorticityBar) {
+    Impl& m = *impl_;
+    PBFSolver::State& s = m.S();
+    if (m.checkpoints.empty() && m.frames != 0)
+        throw std::logic_error("PBFAdjoint::backward needs a forward() first");
+    if (particleCount != m.n || s.count != m.n)
+        throw std::out_of_range("Requested adjoint count does not match active particles");
+    const std::size_t n = m.n;
+    const SimulationParams& P = s.params;
+    for (DeviceBuffer<float4>* b : {&m.pb, &m.vb, &m.x0b, &m.v1b, &m.v3b, &m.v2b, &m.etab, &m.wb,
+                                    &m.prb, &m.gsum})
+        b->allocate(n);
+    for (DeviceBuffer<float>* b : {&m.wlb, &m.cterm, &m.lamb}) b->allocate(n);
+    m.acc.allocate(2);
+    hip_check(hipMemsetAsync(m.acc.data(), 0, 2 * sizeof(double), s.stream));
+    const auto load = [&](DeviceBuffer<float4>& d, const float4* h) {
+        if (h != nullptr)
+            hip_check(hipMemcpyAsync(d.data(), h, n * sizeof(float4), hipMemcpyHostToDevice,
+                                     s.stream));
+        else
+            hip_check(hipMemsetAsync(d.data(), 0, n * sizeof(float4), s.stream));
+    };
+    load(m.pb, xBar);
+    load(m.vb, vBar);
+
+    pbfint::AdjParams ap{};
+    ap.h = P.smoothingRadius;
+    ap.dt = P.dt / static_cast<float>(P.substeps);
+    ap.visc = P.xsphViscosity;
+    ap.vort = P.vorticityStrength;
+    ap.radius = P.particleRadius;
+    ap.restitution = P.collisionRestitution;
+    ap.friction = P.collisionFriction;
+    ap.mass = P.particleMass;
+    ap.rho0 = P.restDensity;
+    ap.lambdaEps = P.lambdaEpsilon;
+    ap.scorrK = P.scorrK;
+    ap.scorrN = P.scorrN;
+    ap.scorrDq = P.scorrDeltaQ;
+    const Container& box = s.collision.state().hostContainer;
+    ap.box = pbf_container{box.min, box.max};
+    const auto build = [&](SubstepTape& tp, int b) {
+        return pbfint::AdjBuild{tp.nbr[b].data(), tp.cnt[b].data(), tp.ids[b].data(), s.stride};
+    };
+
+    const std::size_t segments = m.checkpoints.size();
+    for (std::size_t k = segments; k-- > 0;) {
+        const std::size_t f0 = k * m.every;
+        const std::size_t f1 = std::min(m.frames, f0 + m.every);
+        const std::size_t subs = (f1 - f0) * static_cast<std::size_t>(P.substeps);
+        if (m.tapes.size() < subs) m.tapes.resize(subs);
+        for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(n, s.nbr.size());
+        // Replay the segment with the tape on.
+        m.restore(m.checkpoints[k]);
+        s.reset_status();
+        s.tapes = &m.tapes;
+        s.tapeNext = 0;
+        try {
+            for (std::size_t f = f0; f < f1; ++f) s.enqueue_frame();
+        } catch (...) {
+            s.tapes = nullptr;
+            throw;
+        }
+        s.tapes = nullptr;
+        s.finish();
+        // Back through the segment's substeps.
+        for (std::size_t q = subs; q-- > 0;) {
+            SubstepTape& tp = m.tapes[q];
+            pbfint::AdjBuffers w{m.pb.data(), m.vb.data(), m.x0b.data(), m.v1b.data(),
+                                 m.v3b.data(), m.v2b.data(), m.etab.data(), m.wb.data(),
+                                 m.prb.data(), m.gsum.data(), m.wlb.data(), m.cterm.data(),
+                                 m.lamb.data(), m.acc.data()};
+            const pbfint::AdjPost post{build(tp, 4), tp.xs.data(), tp.pf.data(), tp.v3.data(),
+                                       tp.omega.data(), tp.vin.data()};
+            launch_check(pbfint::adjoint_post(post, w, n, ap, s.stream));
+            for (int it = P.solverIterations; it-- > 0;) {
+                const pbfint::AdjIter iter{build(tp, it), tp.pk[it].data(), tp.pout[it].data()};
+                launch_check(pbfint::adjoint_iteration(iter, w, n, ap, s.stream));
+            }
+            launch_check(pbfint::adjoint_predict(w, n, ap.dt, s.stream));
+            std::swap(m.pb, m.x0b);
+            std::swap(m.vb, m.v1b);
+        }
+    }
+    if (x0Bar != nullptr)
+        hip_check(hipMemcpyAsync(x0Bar, m.pb.data(), n * sizeof(float4), hipMemcpyDeviceToHost,
+                                 s.stream));
+    if (v0Bar != nullptr)
+        hip_check(hipMemcpyAsync(v0Bar, m.vb.data(), n * sizeof(float4), hipMemcpyDeviceToHost,
+                                 s.stream));
+    double acc[2] = {0.0, 0.0};
+    hip_check(hipMemcpyAsync(acc, m.acc.data(), sizeof(acc), hipMemcpyDeviceToHost, s.stream));
+    m.restore(m.last);
+    hip_check(hipStreamSynchronize(s.stream));
+    if (viscosityBar != nullptr) *viscosityBar = acc[0];
+    if (vorticityBar != nullptr) *vorticityBar = acc[1];
 }
 
 }  // namespace pbf

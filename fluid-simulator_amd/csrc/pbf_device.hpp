@@ -438,6 +438,43 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
     }
 }
 
+// Neighbour loop: kNbrBatch list entries, then their kNbrBatch records, are
+// in flight before the first is used, so a wave's gathers overlap instead of
+// waiting one neighbour at a time.  Contributions are still applied in list
+// order, so every sum is formed exactly as the one-neighbour-at-a-time loop
+// forms it.  kGrouped reads the step's grouped lists (four entries per
+// 16-byte load, free slots of the last group repeating its last entry);
+// otherwise the reference's slot-major list, where slots past the count
+// reload the last valid entry (c >= 1 inside the loop).  Slots past the
+// count are not applied.
+constexpr int kNbrBatch = 4;  // a group of the step's lists
+
+template <bool kGrouped, class Load, class Use>
+__device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ nbr, size_t stride,
+                                                   size_t i, int c, Load load, Use use) {
+    for (int q0 = 0; q0 < c; q0 += kNbrBatch) {
+        uint32_t j[kNbrBatch];
+        if (kGrouped) {
+            const uint4* groups = reinterpret_cast<const uint4*>(nbr);
+            const uint4 g = groups[static_cast<size_t>(q0 / 4) * stride + i];
+            j[0] = g.x;
+            j[1] = g.y;
+            j[2] = g.z;
+            j[3] = g.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < kNbrBatch; ++u)
+                j[u] = nbr[static_cast<size_t>(min(q0 + u, c - 1)) * stride + i];
+        }
+        decltype(load(0u)) r[kNbrBatch];
+#pragma unroll
+        for (int u = 0; u < kNbrBatch; ++u) r[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < kNbrBatch; ++u)
+            if (q0 + u < c) use(r[u]);
+    }
+}
+
 // Launch geometry: 256 threads (4 wave64) per block, the reference's kBlockSize.
 constexpr int kBlock = 256;
 

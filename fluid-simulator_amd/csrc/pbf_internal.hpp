@@ -98,6 +98,66 @@ pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_flo
                          uint32_t* ido, size_t n, void* stream);
 pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream);
 
+// Adjoint of the substep (k_adjoint.hip).  One grid build of the forward's
+// tape: its grouped lists and counts, and the particle id of every slot.
+struct AdjBuild {
+    const uint32_t* nbr;
+    const int* cnt;
+    const uint32_t* ids;
+    size_t stride;
+};
+// The post-solve build and the records of that phase (its slot order).
+struct AdjPost {
+    AdjBuild build;
+    const pbf_float4* xs;     // substep start positions
+    const pbf_float4* pf;     // final positions
+    const pbf_float4* v3;     // velocities after XSPH
+    const pbf_float4* omega;  // vorticity, |omega| in .w
+    const pbf_float4* vin;    // velocities after gravity
+};
+// One constraint iteration: positions entering it (lambda in .w) and
+// leaving it (after the container projection), in its build's slot order.
+struct AdjIter {
+    AdjBuild build;
+    const pbf_float4* pk;
+    const pbf_float4* pout;
+};
+// Adjoint state.  pb, vb (the adjoints of a substep's output positions and
+// velocities), x0b, v1b (of its input positions and velocities) are in
+// particle-id order; the rest are per-slot temporaries; acc holds the
+// adjoints of the XSPH and vorticity strengths (FP64).
+struct AdjBuffers {
+    pbf_float4* pb;
+    const pbf_float4* vb;
+    pbf_float4* x0b;
+    pbf_float4* v1b;
+    pbf_float4* v3b;
+    pbf_float4* v2b;
+    pbf_float4* etab;
+    pbf_float4* wb;
+    pbf_float4* prb;
+    pbf_float4* gsum;
+    float* wlb;
+    float* cterm;
+    float* lamb;
+    double* acc;
+};
+struct AdjParams {
+    float h, dt, visc, vort, radius, restitution, friction;
+    float mass, rho0, lambdaEps, scorrK, scorrDq;
+    int scorrN;
+    pbf_container box;
+};
+// Post-solve phase backward: from pb (output positions) and vb (output
+// velocities) to x0b, v1b partially and pb (the last iteration's output).
+pbf_status adjoint_post(const AdjPost& t, const AdjBuffers& w, size_t n, const AdjParams& p,
+                        void* stream);
+// One constraint iteration backward: pb from its output to its input.
+pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n,
+                             const AdjParams& p, void* stream);
+// Integrate backward: completes x0b and v1b from pb.
+pbf_status adjoint_predict(const AdjBuffers& w, size_t n, float dt, void* stream);
+
 // Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
 // have cell size == h (a 3x3x3 stencil) and the working set must be in the
 // order of sortedKeys; tileEnd comes from launch_cell_ranges.

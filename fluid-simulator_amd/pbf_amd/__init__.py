@@ -26,7 +26,7 @@ import numpy as np
 __all__ = [
     "lib", "LIB_PATH", "float3", "int3", "SimulationParams", "Container", "SphereCollider",
     "BoxCollider", "PlaneCollider", "make_float3", "DeviceArray", "PBFSolver", "SpatialGrid",
-    "PBFBatchSolver", "pinned_empty",
+    "PBFBatchSolver", "PBFAdjoint", "pinned_empty",
     "CollisionSystem", "kernels", "PbfError", "InvalidArgument", "LogicError", "OutOfRange",
     "LengthError", "NeighborOverflow", "CollisionNonConvergence", "DeviceError",
     "STATUS_NEIGHBOR_OVERFLOW", "STATUS_COLLISION_NONCONVERGENCE",
@@ -249,6 +249,11 @@ _SIGNATURES = {
     "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
     "pbf_solver_neighbor_lists": (_st, [_vp, _vp, _vp, _vp, _sz, _i]),
+    "pbf_adjoint_create": (_st, [_vp, C.POINTER(_vp)]),
+    "pbf_adjoint_destroy": (_st, [_vp]),
+    "pbf_adjoint_forward": (_st, [_vp, _sz, _sz]),
+    "pbf_adjoint_backward": (_st, [_vp, _vp, _vp, _sz, _vp, _vp, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
     "pbf_batch_create": (_st, [C.POINTER(_vp)]),
     "pbf_batch_destroy": (_st, [_vp]),
     "pbf_batch_initialize": (_st, [_vp, _vp, _sz, _vp]),
@@ -735,4 +740,38 @@ class PBFBatchSolver:
                                             C.byref(count)))
         keys = names.value.decode().split("\n")[:count.value]
         return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
+
+
+class PBFAdjoint:
+    """Reverse-mode derivatives of a PBFSolver's frames (pbf_adjoint_*).
+
+    forward(frames, every) runs the solver and keeps checkpoints;
+    backward(xbar, vbar) returns (x0bar, v0bar, viscosity_bar,
+    vorticity_bar) for dL/dx, dL/dv of the final state given in API order.
+    Neighbour lists and the forward's discrete decisions are held fixed."""
+
+    def __init__(self, solver: "PBFSolver"):
+        self._solver = solver  # keeps the solver alive
+        h = _vp()
+        _check(lib().pbf_adjoint_create(solver._h, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pbf_adjoint_destroy(self._h)
+            self._h = None
+
+    def forward(self, frames: int, checkpoint_every: int = 10):
+        _check(lib().pbf_adjoint_forward(self._h, int(frames), int(checkpoint_every)))
+
+    def backward(self, xbar=None, vbar=None):
+        n = self._solver.particleCount()
+        xb = None if xbar is None else as_float4(xbar)
+        vb = None if vbar is None else as_float4(vbar)
+        x0b = np.zeros((n, 4), np.float32)
+        v0b = np.zeros((n, 4), np.float32)
+        cb, eb = C.c_double(), C.c_double()
+        _check(lib().pbf_adjoint_backward(self._h, _ptr(xb), _ptr(vb), n, _ptr(x0b), _ptr(v0b),
+                                          C.byref(cb), C.byref(eb)))
+        return x0b, v0b, cb.value, eb.value
 

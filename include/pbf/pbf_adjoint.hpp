@@ -1,0 +1,49 @@
+// Reverse-mode derivatives of PBFSolver frames.  No counterpart in the
+// reference (its solver has no derivatives).
+//
+// forward() runs frames of a solver and keeps its working set every
+// `checkpointEvery` frames; backward() replays each checkpoint segment with
+// a tape (neighbour lists, orderings and per-pass records of every
+// substep) and runs the hand-written adjoint kernels back through it, so
+// memory holds one segment's tape at a time.  Neighbour lists and every
+// discrete decision of the forward (container projections, wall contacts)
+// are held fixed; between them the derivatives are exact up to FP32
+// rounding.  Models the container only (no spheres, boxes or planes) and
+// needs vorticity confinement enabled (the post-solve pass it records).
+#pragma once
+
+#include <cstddef>
+#include <memory>
+
+#include "pbf/pbf_solver.hpp"
+#include "pbf/types.hpp"
+
+namespace pbf {
+
+class PBFAdjoint {
+public:
+    // The solver must outlive this object and keep its particles and
+    // parameters between forward() and backward().
+    explicit PBFAdjoint(PBFSolver& solver);
+    ~PBFAdjoint();
+    PBFAdjoint(const PBFAdjoint&) = delete;
+    PBFAdjoint& operator=(const PBFAdjoint&) = delete;
+
+    // Runs frameCount frames (as PBFSolver::run) from the solver's current
+    // state, keeping a checkpoint every checkpointEvery frames.
+    void forward(std::size_t frameCount, std::size_t checkpointEvery);
+
+    // Given dL/dx and dL/dv of the state forward() ended in (host arrays in
+    // API order, particleCount records; null means zero), writes dL/dx and
+    // dL/dv of the state it started from, and dL/d(xsphViscosity) and
+    // dL/d(vorticityStrength) (any output may be null).  The solver is left
+    // in the state forward() ended in.
+    void backward(const float4* xBar, const float4* vBar, std::size_t particleCount,
+                  float4* x0Bar, float4* v0Bar, double* viscosityBar, double* vorticityBar);
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace pbf

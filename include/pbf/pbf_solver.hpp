@@ -75,6 +75,7 @@ public:
 
 private:
     friend class PBFBatchSolver;  // runs the same step over a batch of scenes
+    friend class PBFAdjoint;      // replays frames with a tape
     std::unique_ptr<State> state_;
 };
 

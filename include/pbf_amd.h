@@ -426,6 +426,29 @@ pbf_status pbf_batch_kernel_times(pbf_batch* batch, char* names, size_t namesByt
                                   double* milliseconds, uint64_t* launches, size_t capacity,
                                   size_t* kernelCount);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint (PBFAdjoint, include/pbf/pbf_adjoint.hpp).  No counterpart in the */
+/* reference: reverse-mode derivatives of a solver's frames, with neighbour */
+/* lists and the forward's discrete decisions held fixed, checkpointed      */
+/* every checkpointEvery frames and replayed with a tape per segment.       */
+/* Container only; vorticity confinement must be enabled.                  */
+/* ------------------------------------------------------------------------ */
+typedef struct pbf_adjoint pbf_adjoint;
+
+/* The solver must outlive the adjoint. */
+pbf_status pbf_adjoint_create(pbf_solver* solver, pbf_adjoint** adjoint);
+pbf_status pbf_adjoint_destroy(pbf_adjoint* adjoint);
+/* Runs frameCount frames of the solver, keeping checkpoints. */
+pbf_status pbf_adjoint_forward(pbf_adjoint* adjoint, size_t frameCount, size_t checkpointEvery);
+/* dL/dx, dL/dv of the final state in (host, API order; NULL = zero), dL/dx,
+ * dL/dv of the initial state and dL/d(xsphViscosity),
+ * dL/d(vorticityStrength) out (any may be NULL).  The solver is left in the
+ * state pbf_adjoint_forward ended in. */
+pbf_status pbf_adjoint_backward(pbf_adjoint* adjoint, const pbf_float4* h_xBar,
+                                const pbf_float4* h_vBar, size_t particleCount,
+                                pbf_float4* h_x0Bar, pbf_float4* h_v0Bar, double* viscosityBar,
+                                double* vorticityBar);
+
 #ifdef __cplusplus
 }
 #endif

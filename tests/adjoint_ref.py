@@ -330,3 +330,43 @@ def backward(aux, pairs, P, c, e, xb, vb):
     xb_out = xb0 + pb
     v1b += pb * P.dt
     return xb_out, v1b, cb_x, eb
+
+
+# ---------------------------------------------------------------------------
+# Full FP64 forward: neighbour pairs rebuilt at every grid build
+# ---------------------------------------------------------------------------
+def pairs_within(p, h):
+    """Directed pairs (i, j), i != j, with |p_i - p_j|^2 <= h^2 (the radius
+    test of findNeighbors; the 27-cell stencil holds every such pair when
+    the cell size is h)."""
+    from scipy.spatial import cKDTree
+
+    und = cKDTree(p).query_pairs(h * (1.0 + 1e-12), output_type="ndarray")
+    if len(und) == 0:
+        return np.zeros(0, int), np.zeros(0, int)
+    d = p[und[:, 0]] - p[und[:, 1]]
+    und = und[np.einsum("ij,ij->i", d, d) <= h * h]
+    I = np.concatenate([und[:, 0], und[:, 1]])
+    J = np.concatenate([und[:, 1], und[:, 0]])
+    return I, J
+
+
+def substep_full(x, v, P, c, e):
+    """One substep in FP64 with the pairs of every build taken from the
+    positions entering it: the function central differences see."""
+    g = np.array(P.gravity)
+    p = x + (v + g * P.dt) * P.dt
+    pairs = []
+    for _ in range(4):
+        pairs.append(pairs_within(p, P.h))
+        L = lambda_terms(p, *pairs[-1], P)
+        p, _ = container(p + delta_p(p, L["lam"], *pairs[-1], P), P)
+    pairs.append(pairs_within(p, P.h))
+    return forward(x, v, pairs, P, c, e)
+
+
+def run_full(x, v, substeps, P, c, e):
+    for _ in range(substeps):
+        x, v = substep_full(x, v, P, c, e)
+    return x, v
+

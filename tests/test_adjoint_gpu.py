@@ -1,0 +1,129 @@
+"""The library's adjoint (pbf_adjoint_*) against FP64 central differences.
+
+The differences are taken of a genuinely FP64 forward (adjoint_ref.run_full:
+every state in FP64, neighbour pairs rebuilt at every grid build, the
+container projection and contact velocities live), so they see the true
+derivative of the step; the adjoint holds the forward's lists and discrete
+decisions fixed, which is the same derivative wherever no decision sits
+within the difference step.  Loss: the mean squared distance of the final
+positions to the initial block moved by +0.2 in x.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import adjoint_ref as A
+from pbf_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nx, substeps, seed=5):
+    sc = scenes.small_block(nx, nx, nx, substeps=substeps)
+    P = sc.params
+    P.xsphViscosity = 1e-4
+    P.vorticityStrength = 1e-4
+    rng = np.random.default_rng(seed)
+    v0 = sc.velocities.copy()
+    v0[:, :3] = rng.normal(0.0, 0.1, (sc.n, 3)).astype(np.float32)
+    Q = A.Params(dt=P.dt / P.substeps, h=P.smoothingRadius, mass=P.particleMass,
+                 rho0=P.restDensity, eps_lambda=P.lambdaEpsilon,
+                 gravity=(P.gravity.x, P.gravity.y, P.gravity.z), scorr_k=P.scorrK,
+                 scorr_n=P.scorrN, scorr_dq=P.scorrDeltaQ, radius=P.particleRadius,
+                 box_min=sc.minBounds, box_max=sc.maxBounds)
+    return sc, P, Q, v0, rng
+
+
+@pytest.mark.parametrize("nx,substeps,frames,every", [(10, 1, 4, 2), (12, 2, 5, 2)])
+def test_adjoint_matches_fp64_central_differences(gpu, nx, substeps, frames, every):
+    import pbf_amd
+
+    sc, P, Q, v0, rng = _case(nx, substeps)
+    x0 = sc.positions
+    target = x0[:, :3].astype(np.float64) + np.array([0.2, 0.0, 0.0])
+
+    def loss(x):
+        d = x - target
+        return np.mean(np.sum(d * d, 1))
+
+    s = gpu.Solver()
+    s.initialize(sc.n, sc.minBounds, sc.maxBounds, P)
+    s.setParticles(x0, v0)
+    adj = pbf_amd.PBFAdjoint(s)
+    adj.forward(frames, every)
+    xT = s.copyPositionsToHost()
+    xbar = np.zeros_like(xT)
+    xbar[:, :3] = (2.0 * (xT[:, :3] - target) / sc.n).astype(np.float32)
+    _, v0b, cb, eb = adj.backward(xbar, None)
+    # backward leaves the solver where forward ended
+    assert np.array_equal(s.copyPositionsToHost(), xT)
+
+    x = x0[:, :3].astype(np.float64)
+    v = v0[:, :3].astype(np.float64)
+    S = frames * substeps
+    c, e = P.xsphViscosity, P.vorticityStrength
+
+    def L64(v_, c_=c, e_=e):
+        return loss(A.run_full(x, v_, S, Q, c_, e_)[0])
+
+    assert abs(L64(v) - loss(xT[:, :3].astype(np.float64))) < 1e-6 * L64(v)
+    fd, ad = [], []
+    for k in rng.choice(sc.n, 5, replace=False):
+        for d in range(3):
+            vp, vm = v.copy(), v.copy()
+            vp[k, d] += 1e-4
+            vm[k, d] -= 1e-4
+            fd.append((L64(vp) - L64(vm)) / 2e-4)
+            ad.append(float(v0b[k, d]))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    print(f"[{sc.n} particles, {S} substeps] v0: {len(fd)} coordinates, relative error {err:.2e}",
+          flush=True)
+    assert err <= 1e-3
+    # The parameters move every particle, so a difference step of 1e-3 or
+    # 1e-2 of their value already carries some contact decisions across
+    # their switch (the loss of the two-substep case is piecewise smooth in
+    # c: differences of -5.30, -5.90, -5.265081, -5.265085 at steps of
+    # 1e-2 .. 1e-5 relative, against the adjoint's -5.265085); 1e-5 stays
+    # inside one smooth piece.
+    for name, base, grad in (("xsphViscosity", c, cb), ("vorticityStrength", e, eb)):
+        hh = 1e-5 * base
+        f = ((L64(v, c_=c + hh) - L64(v, c_=c - hh)) if name == "xsphViscosity"
+             else (L64(v, e_=e + hh) - L64(v, e_=e - hh))) / (2 * hh)
+        print(f"  d/d{name}: fd {f:+.6e} adjoint {grad:+.6e}", flush=True)
+        assert abs(f - grad) <= 1e-3 * abs(f)
+
+
+def test_torch_rollout_gradients_equal_the_adjoint(gpu):
+    """torch.autograd through pbf_amd.autograd.rollout hands back exactly
+    the adjoint's gradients for a loss built from torch operations."""
+    import torch
+
+    import pbf_amd
+    from pbf_amd.autograd import rollout
+
+    sc, P, Q, v0, rng = _case(8, 1)
+    target = torch.from_numpy(sc.positions[:, :3].astype(np.float64)) + torch.tensor([0.2, 0.0, 0.0])
+    v = torch.from_numpy(v0[:, :3].astype(np.float64)).requires_grad_(True)
+    c = torch.tensor(P.xsphViscosity, dtype=torch.float64, requires_grad=True)
+    e = torch.tensor(P.vorticityStrength, dtype=torch.float64, requires_grad=True)
+    xT = rollout(v, c, e, sc, 3, 2)
+    loss = ((xT - target) ** 2).sum(1).mean()
+    loss.backward()
+
+    s = gpu.Solver()
+    s.initialize(sc.n, sc.minBounds, sc.maxBounds, P)
+    s.setParticles(sc.positions, v0)
+    adj = pbf_amd.PBFAdjoint(s)
+    adj.forward(3, 2)
+    x = s.copyPositionsToHost()
+    xbar = np.zeros_like(x)
+    xbar[:, :3] = (2.0 * (x[:, :3] - target.numpy()) / sc.n).astype(np.float32)
+    _, v0b, cb, eb = adj.backward(xbar, None)
+    # two runs of the adjoint sum through atomics in different orders: equal
+    # to rounding, not bitwise
+    g = v0b[:, :3].astype(np.float64)
+    assert np.abs(v.grad.numpy() - g).max() <= 1e-4 * np.abs(g).max()
+    assert abs(c.grad.item() - cb) <= 1e-4 * abs(cb)
+    assert abs(e.grad.item() - eb) <= 1e-4 * abs(eb)
