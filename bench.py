@@ -30,6 +30,9 @@ Prints one JSON line:
                  particles, about 10M in all): one PBFBatchSolver against a
                  loop of PBFSolver.step() over one solver per scene, and the
                  speedup (--no-batched skips it)
+  adjoint        the differentiable workload (65,536 particles, 60 frames,
+                 checkpoints every 10): the adjoint's backward pass against a
+                 plain run of the same frames (--no-adjoint skips it)
   cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on
                  the host cores, on a bounded sample of the same workload
 
@@ -219,6 +222,41 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
             "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
 
 
+def adjoint_cost(pbf_amd, scenes, frames: int = 60, every: int = 10) -> dict:
+    """The differentiable workload (65,536 particles, 60 frames, checkpoints
+    every 10): forward() against backward() (the taped replay of every
+    checkpoint segment plus the adjoint kernels), and a plain run of the
+    same frames."""
+    import numpy as np
+
+    sc, target = scenes.differentiable_block()
+    s = pbf_amd.PBFSolver()
+    s.initialize(sc.n, sc.minBounds, sc.maxBounds, sc.params)
+    s.setParticles(sc.positions, sc.velocities)
+    s.run(2)  # first-use costs out of the timings
+    s.setParticles(sc.positions, sc.velocities)
+    t0 = time.perf_counter()
+    s.run(frames)
+    pbf_amd.synchronize()
+    plain = time.perf_counter() - t0
+    s.setParticles(sc.positions, sc.velocities)
+    adj = pbf_amd.PBFAdjoint(s)
+    t0 = time.perf_counter()
+    adj.forward(frames, every)
+    pbf_amd.synchronize()
+    fwd = time.perf_counter() - t0
+    x = s.copyPositionsToHost()
+    xbar = np.zeros_like(x)
+    xbar[:, :3] = (2.0 * (x[:, :3] - target) / sc.n).astype(np.float32)
+    t0 = time.perf_counter()
+    adj.backward(xbar, None)
+    bwd = time.perf_counter() - t0
+    return {"workload": f"{sc.name}: {sc.n} particles, {frames} frames of "
+                        f"{sc.params.substeps} substep(s), checkpoints every {every}",
+            "run_ms": plain * 1e3, "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+            "backward_over_run": bwd / plain}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +268,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-versus-looped measurement (256 scenes)")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint cost measurement (65,536 particles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
     a = ap.parse_args()
 
@@ -306,6 +346,8 @@ def main() -> int:
     if not a.no_batched:
         batched = batched_vs_looped(pbf_amd, scenes, a.batch_scenes, frames=3)
 
+    adjoint = None if a.no_adjoint else adjoint_cost(pbf_amd, scenes)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene)
@@ -336,6 +378,7 @@ def main() -> int:
         "step_roofline": step_roof,
         "b_alg_roofline": b_alg_roof,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "kernels": kernel_rows,
     }

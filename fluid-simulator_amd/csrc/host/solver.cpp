@@ -1046,8 +1046,8 @@ struct PBFAdjoint::Impl {
     std::vector<Checkpoint> checkpoints;  // frame 0, every, 2 every, ...
     Checkpoint last;                      // the state forward() ended in
     std::vector<SubstepTape> tapes;
-    DeviceBuffer<float4> pb, vb, x0b, v1b, v3b, v2b, etab, wb, prb, gsum;
-    DeviceBuffer<float> wlb, cterm, lamb;
+    DeviceBuffer<float4> pb, vb, x0b, v1b, v3b, etab, wb, prb, gsum;
+    DeviceBuffer<float> cterm, lamb;
     DeviceBuffer<double> acc;
 
     PBFSolver::State& S() { return *solver->state_; }
@@ -1115,10 +1115,10 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
         throw std::out_of_range("Requested adjoint count does not match active particles");
     const std::size_t n = m.n;
     const SimulationParams& P = s.params;
-    for (DeviceBuffer<float4>* b : {&m.pb, &m.vb, &m.x0b, &m.v1b, &m.v3b, &m.v2b, &m.etab, &m.wb,
-                                    &m.prb, &m.gsum})
+    for (DeviceBuffer<float4>* b :
+         {&m.pb, &m.vb, &m.x0b, &m.v1b, &m.v3b, &m.etab, &m.wb, &m.prb, &m.gsum})
         b->allocate(n);
-    for (DeviceBuffer<float>* b : {&m.wlb, &m.cterm, &m.lamb}) b->allocate(n);
+    for (DeviceBuffer<float>* b : {&m.cterm, &m.lamb}) b->allocate(n);
     m.acc.allocate(2);
     hip_check(hipMemsetAsync(m.acc.data(), 0, 2 * sizeof(double), s.stream));
     const auto load = [&](DeviceBuffer<float4>& d, const float4* h) {
@@ -1174,10 +1174,9 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
         // Back through the segment's substeps.
         for (std::size_t q = subs; q-- > 0;) {
             SubstepTape& tp = m.tapes[q];
-            pbfint::AdjBuffers w{m.pb.data(), m.vb.data(), m.x0b.data(), m.v1b.data(),
-                                 m.v3b.data(), m.v2b.data(), m.etab.data(), m.wb.data(),
-                                 m.prb.data(), m.gsum.data(), m.wlb.data(), m.cterm.data(),
-                                 m.lamb.data(), m.acc.data()};
+            pbfint::AdjBuffers w{m.pb.data(),   m.vb.data(),    m.x0b.data(),  m.v1b.data(),
+                                 m.v3b.data(),  m.etab.data(),  m.wb.data(),   m.prb.data(),
+                                 m.gsum.data(), m.cterm.data(), m.lamb.data(), m.acc.data()};
             const pbfint::AdjPost post{build(tp, 4), tp.xs.data(), tp.pf.data(), tp.v3.data(),
                                        tp.omega.data(), tp.vin.data()};
             launch_check(pbfint::adjoint_post(post, w, n, ap, s.stream));

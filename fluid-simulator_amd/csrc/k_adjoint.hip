@@ -9,11 +9,12 @@
 // grouped lists and counts, and the positions of that build's working-set
 // order.  Adjoint accumulators of positions and velocities are kept in
 // particle-id (API) order; per-build temporaries in that build's slot order.
-// A thread owns one slot and loops over its list; what a pair contributes to
-// the neighbour goes out by hardware float atomics (the neighbour's own
-// thread could pull it instead, at the price of a second loop over the
-// same pairs).  Sums therefore run in no fixed order: the adjoint is
-// deterministic only to rounding.
+// A thread owns one slot and loops over its list once per pass, pulling
+// both what its own pair terms owe it and what its neighbours' terms owe it:
+// the lists are symmetric (j lists i exactly when i lists j), G is odd in
+// d and its Jacobian even, so the neighbour's term is a function of the
+// neighbour's per-slot records and the same d.  No pair term goes through an
+// atomic; only the two parameter sums do (per wave, in FP64).
 //
 // Reference of the differentiated forward: src/pbf/pbf_solver.cu:186-328,
 // src/pbf/pbf_solver_kernels.cu:26-385, src/pbf/collision_kernels.cu:21-82,
@@ -39,13 +40,6 @@ __device__ __forceinline__ float3 operator*(float s, float3 a) {
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
     return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-
-__device__ __forceinline__ void atomic_add3(pbf_float4* base, uint32_t i, float3 v) {
-    float* p = reinterpret_cast<float*>(base + i);
-    unsafeAtomicAdd(p + 0, v.x);
-    unsafeAtomicAdd(p + 1, v.y);
-    unsafeAtomicAdd(p + 2, v.z);
 }
 
 // Sum over the wave, then one double atomic per wave.
@@ -126,7 +120,7 @@ struct PostArgs {
 __global__ void __launch_bounds__(kBlock)
     k_adj_post1(PostArgs a, const pbf_float4* __restrict__ vb, pbf_float4* __restrict__ v1b,
                 pbf_float4* __restrict__ v3b, pbf_float4* __restrict__ etab,
-                pbf_float4* __restrict__ wb, float* __restrict__ wlb, double* __restrict__ acc) {
+                pbf_float4* __restrict__ wb, double* __restrict__ acc) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     float epart = 0.0f;
     if (i < a.n) {
@@ -188,56 +182,46 @@ __global__ void __launch_bounds__(kBlock)
         const float3 eb = ok ? (1.0f / el) * (Nb - dot3(N, Nb) * N) : make_float3(0.0f, 0.0f, 0.0f);
         st4(etab, i, make_float4(eb.x, eb.y, eb.z, 0.0f));
         st4(wb, i, make_float4(wbar.x, wbar.y, wbar.z, 0.0f));
-        wlb[i] = 0.0f;
     }
     wave_add(acc + 1, epart);
 }
 
-// eta_i = sum_j (|w_j| - |w_i|) G(d_ij): adjoints of |w| and of positions.
+// eta_i = sum_j (|w_j| - |w_i|) G(d_ij).  Pulled: |w|-bar_i = -sum_j
+// (eta-bar_i + eta-bar_j) . G_ij, and d-bar_ij - d-bar_ji = a_ij J (eta-bar_i +
+// eta-bar_j); then |w|-bar into w-bar.
 __global__ void __launch_bounds__(kBlock)
-    k_adj_post2(PostArgs a, const pbf_float4* __restrict__ etab, float* __restrict__ wlb,
+    k_adj_post2(PostArgs a, const pbf_float4* __restrict__ etab, pbf_float4* __restrict__ wb,
                 pbf_float4* __restrict__ pb) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const Sph s = sph_consts(a.h);
     const float3 pi = f3(ld4(a.pf, i));
-    const float wli = a.om[i].w;
-    const float3 eb = f3(ld4(etab, i));
-    float wacc = 0.0f;
+    const float4 wi = ld4(a.om, i);
+    const float3 ebi = f3(ld4(etab, i));
+    float wl = 0.0f;
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
-    const uint32_t* ids = a.b.ids;
     for_each_neighbour<true>(
         a.b.nbr, a.b.stride, i, a.b.cnt[i],
         [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.om, j)}; },
         [&](const Nbr2& r) {
-            const uint32_t j = r.j;
             const float3 d = pi - f3(r.a);
-            const float3 G = spiky(d, s);
-            const float gdot = dot3(eb, G);
-            unsafeAtomicAdd(wlb + j, gdot);
-            wacc -= gdot;
-            const float3 db = (r.b.w - wli) * spiky_jac(d, eb, s);
-            pacc = pacc + db;
-            atomic_add3(pb, ids[j], make_float3(-db.x, -db.y, -db.z));
+            const float3 eb = ebi + f3(ld4(etab, r.j));
+            wl -= dot3(eb, spiky(d, s));
+            pacc = pacc + (r.b.w - wi.w) * spiky_jac(d, eb, s);
         });
-    unsafeAtomicAdd(wlb + i, wacc);
-    atomic_add3(pb, ids[i], pacc);
-}
-
-// |w|-bar into w-bar; clears the XSPH accumulator.
-__global__ void __launch_bounds__(kBlock)
-    k_adj_post3(PostArgs a, const float* __restrict__ wlb, pbf_float4* __restrict__ wb,
-                pbf_float4* __restrict__ v2b) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const float4 wi = ld4(a.om, i);
     float3 w = f3(ld4(wb, i));
-    if (wi.w > 0.0f) w = w + (wlb[i] / wi.w) * f3(wi);
+    if (wi.w > 0.0f) w = w + (wl / wi.w) * f3(wi);
     st4(wb, i, make_float4(w.x, w.y, w.z, 0.0f));
-    st4(v2b, i, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    const uint32_t gid = a.b.ids[i];
+    float4 p = ld4(pb, gid);
+    p.x += pacc.x;
+    p.y += pacc.y;
+    p.z += pacc.z;
+    st4(pb, gid, p);
 }
 
-// w_i = sum_j G(d_ij) x (v3_j - v3_i).
+// w_i = sum_j G(d_ij) x (v3_j - v3_i).  Pulled: v3-bar_i -= sum_j (w-bar_i +
+// w-bar_j) x G_ij, and d-bar_ij - d-bar_ji = J (u_ij x (w-bar_i + w-bar_j)).
 __global__ void __launch_bounds__(kBlock)
     k_adj_post4(PostArgs a, const pbf_float4* __restrict__ wb, pbf_float4* __restrict__ v3b,
                 pbf_float4* __restrict__ pb) {
@@ -246,33 +230,34 @@ __global__ void __launch_bounds__(kBlock)
     const Sph s = sph_consts(a.h);
     const float3 pi = f3(ld4(a.pf, i));
     const float3 vi = f3(ld4(a.v3, i));
-    const float3 wbar = f3(ld4(wb, i));
-    float3 vacc = make_float3(0.0f, 0.0f, 0.0f);
+    const float3 wbi = f3(ld4(wb, i));
+    float3 vacc = f3(ld4(v3b, i));
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
-    const uint32_t* ids = a.b.ids;
     for_each_neighbour<true>(
         a.b.nbr, a.b.stride, i, a.b.cnt[i],
         [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.v3, j)}; },
         [&](const Nbr2& r) {
-            const uint32_t j = r.j;
             const float3 d = pi - f3(r.a);
-            const float3 G = spiky(d, s);
-            const float3 u = f3(r.b) - vi;
-            const float3 ub = cross3(wbar, G);
-            atomic_add3(v3b, j, ub);
-            vacc = vacc - ub;
-            const float3 db = spiky_jac(d, cross3(u, wbar), s);
-            pacc = pacc + db;
-            atomic_add3(pb, ids[j], make_float3(-db.x, -db.y, -db.z));
+            const float3 wbs = wbi + f3(ld4(wb, r.j));
+            vacc = vacc - cross3(wbs, spiky(d, s));
+            pacc = pacc + spiky_jac(d, cross3(f3(r.b) - vi, wbs), s);
         });
-    atomic_add3(v3b, static_cast<uint32_t>(i), vacc);
-    atomic_add3(pb, ids[i], pacc);
+    st4(v3b, i, make_float4(vacc.x, vacc.y, vacc.z, 0.0f));
+    const uint32_t gid = a.b.ids[i];
+    float4 p = ld4(pb, gid);
+    p.x += pacc.x;
+    p.y += pacc.y;
+    p.z += pacc.z;
+    st4(pb, gid, p);
 }
 
-// v3_i = v2_i + c sum_j (v2_j - v2_i) W(d_ij), v2 = (p - x) / dt.
+// v3_i = v2_i + c sum_j (v2_j - v2_i) W_ij with v2 = (p - x) / dt.  Pulled:
+// v2-bar_i = v3-bar_i + c sum_j W_ij (v3-bar_j - v3-bar_i) and d-bar_ij -
+// d-bar_ji = c ((v2_j - v2_i) . (v3-bar_i - v3-bar_j)) grad W_ij; then v2's
+// adjoint into the positions' (+1/dt) and the start positions' (-1/dt).
 __global__ void __launch_bounds__(kBlock)
-    k_adj_post5(PostArgs a, const pbf_float4* __restrict__ v3b, pbf_float4* __restrict__ v2b,
-                pbf_float4* __restrict__ pb, double* __restrict__ acc) {
+    k_adj_post5(PostArgs a, const pbf_float4* __restrict__ v3b, pbf_float4* __restrict__ pb,
+                pbf_float4* __restrict__ x0b, double* __restrict__ acc) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     float cpart = 0.0f;
     if (i < a.n) {
@@ -280,48 +265,32 @@ __global__ void __launch_bounds__(kBlock)
         const float inv_dt = 1.0f / a.dt;
         const float3 pi = f3(ld4(a.pf, i));
         const float3 v2i = inv_dt * (pi - f3(ld4(a.xs, i)));
-        const float3 vb3 = f3(ld4(v3b, i));
-        float3 vacc = vb3;
+        const float3 vbi = f3(ld4(v3b, i));
+        float3 v2b = vbi;
         float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
-        const uint32_t* ids = a.b.ids;
         for_each_neighbour<true>(
             a.b.nbr, a.b.stride, i, a.b.cnt[i],
             [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.xs, j)}; },
             [&](const Nbr2& r) {
-                const uint32_t j = r.j;
                 const float3 pj = f3(r.a);
                 const float3 d = pi - pj;
                 const float W = poly6_d2(dot3(d, d), s);
                 const float3 dv = inv_dt * (pj - f3(r.b)) - v2i;
-                const float3 t = (a.visc * W) * vb3;
-                atomic_add3(v2b, j, t);
-                vacc = vacc - t;
-                cpart += W * dot3(vb3, dv);
-                const float3 db = (a.visc * dot3(dv, vb3)) * poly6_grad(d, s);
-                pacc = pacc + db;
-                atomic_add3(pb, ids[j], make_float3(-db.x, -db.y, -db.z));
+                const float3 vbj = f3(ld4(v3b, r.j));
+                v2b = v2b + (a.visc * W) * (vbj - vbi);
+                cpart += W * dot3(vbi, dv);
+                pacc = pacc + (a.visc * dot3(dv, vbi - vbj)) * poly6_grad(d, s);
             });
-        atomic_add3(v2b, static_cast<uint32_t>(i), vacc);
-        atomic_add3(pb, ids[i], pacc);
+        const uint32_t gid = a.b.ids[i];
+        const float3 g = inv_dt * v2b;
+        float4 p = ld4(pb, gid);
+        p.x += pacc.x + g.x;
+        p.y += pacc.y + g.y;
+        p.z += pacc.z + g.z;
+        st4(pb, gid, p);
+        st4(x0b, gid, make_float4(-g.x, -g.y, -g.z, 0.0f));
     }
     wave_add(acc + 0, cpart);
-}
-
-// v2 = (p - x) / dt: into the position adjoint and the start positions'.
-__global__ void __launch_bounds__(kBlock)
-    k_adj_post6(PostArgs a, const pbf_float4* __restrict__ v2b, pbf_float4* __restrict__ pb,
-                pbf_float4* __restrict__ x0b) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const uint32_t gid = a.b.ids[i];
-    const float inv_dt = 1.0f / a.dt;
-    const float3 g = inv_dt * f3(ld4(v2b, i));
-    float4 p = ld4(pb, gid);
-    p.x += g.x;
-    p.y += g.y;
-    p.z += g.z;
-    st4(pb, gid, p);
-    st4(x0b, gid, make_float4(-g.x, -g.y, -g.z, 0.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -341,8 +310,7 @@ struct IterArgs {
 // in FP64, as the forward) for the next two passes.
 __global__ void __launch_bounds__(kBlock)
     k_adj_it1(IterArgs a, pbf_float4* __restrict__ pb, pbf_float4* __restrict__ prb,
-              pbf_float4* __restrict__ gsum, float* __restrict__ cterm,
-              float* __restrict__ lamb) {
+              pbf_float4* __restrict__ gsum, float* __restrict__ cterm) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t gid = a.b.ids[i];
@@ -376,10 +344,11 @@ __global__ void __launch_bounds__(kBlock)
     st4(gsum, i, make_float4(static_cast<float>(gx), static_cast<float>(gy),
                              static_cast<float>(gz), static_cast<float>(sg)));
     cterm[i] = static_cast<float>(rho / static_cast<double>(a.rho0) - 1.0);
-    lamb[i] = 0.0f;
 }
 
-// Delta-p_i = (m / rho0) sum_j (l_i + l_j + s_ij) G(d_ij).
+// Delta-p_i = (m / rho0) sum_j (l_i + l_j + s_ij) G_ij.  Pulled, with r the
+// projected adjoint: l-bar_i = gs sum_j (r_i - r_j) . G_ij and d-bar_ij -
+// d-bar_ji = gs coef_ij J (r_i - r_j) + gs ((r_i - r_j) . G_ij) grad s_ij.
 __global__ void __launch_bounds__(kBlock)
     k_adj_it2(IterArgs a, const pbf_float4* __restrict__ prb, float* __restrict__ lamb,
               pbf_float4* __restrict__ pb) {
@@ -390,31 +359,41 @@ __global__ void __launch_bounds__(kBlock)
     const float gs = a.mass / a.rho0;
     const float4 pk = ld4(a.pk, i);
     const float3 pi = f3(pk);
-    const float3 r = f3(ld4(prb, i));
-    float lacc = 0.0f;
+    const float3 ri = f3(ld4(prb, i));
+    float lb = 0.0f;
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
-    const uint32_t* ids = a.b.ids;
     for_each_neighbour<true>(
-        a.b.nbr, a.b.stride, i, a.b.cnt[i], [&](uint32_t j) { return Nbr2{j, ld4(a.pk, j), {}}; },
-        [&](const Nbr2& rr) {
-            const uint32_t j = rr.j;
-            const float4 pj = rr.a;
-            const float3 d = pi - f3(pj);
+        a.b.nbr, a.b.stride, i, a.b.cnt[i],
+        [&](uint32_t j) { return Nbr2{j, ld4(a.pk, j), ld4(prb, j)}; },
+        [&](const Nbr2& r) {
+            const float3 d = pi - f3(r.a);
             const float d2 = dot3(d, d);
             const float3 G = spiky(d, s);
-            const float coef = pk.w + pj.w + scorr_value(d2, s.h2, sc);
-            const float q = gs * dot3(r, G);
-            lacc += q;
-            unsafeAtomicAdd(lamb + j, q);
-            const float3 db = (gs * coef) * spiky_jac(d, r, s) + q * scorr_grad(d, s.h2, sc);
-            pacc = pacc + db;
-            atomic_add3(pb, ids[j], make_float3(-db.x, -db.y, -db.z));
+            const float3 rr = ri - f3(r.b);
+            const float q = gs * dot3(rr, G);
+            lb += q;
+            const float coef = pk.w + r.a.w + scorr_value(d2, s.h2, sc);
+            pacc = pacc + (gs * coef) * spiky_jac(d, rr, s) + q * scorr_grad(d, s.h2, sc);
         });
-    unsafeAtomicAdd(lamb + i, lacc);
-    atomic_add3(pb, ids[i], pacc);
+    lamb[i] = lb;
+    const uint32_t gid = a.b.ids[i];
+    float4 p = ld4(pb, gid);
+    p.x += pacc.x;
+    p.y += pacc.y;
+    p.z += pacc.z;
+    st4(pb, gid, p);
 }
 
-// lambda_i = -C_i / (S_i + eps).
+// C-bar and S-bar of lambda_i = -C_i / (S_i + eps).
+__device__ __forceinline__ float2 lambda_bars(float lb, float C, float S, float eps) {
+    const float den = S + eps;
+    return make_float2(-lb / den, lb * C / (den * den));
+}
+
+// lambda_i = -C_i / (S_i + eps), C_i = rho_i / rho0 - 1, S_i = sum_j |g_ij|^2 +
+// |sum_j g_ij|^2, g_ij = -gs G_ij.  Pulled: d-bar_ij - d-bar_ji = gs (C-bar_i
+// + C-bar_j) grad W_ij - gs J (gbar_ij - gbar_ji), gbar_ij = 2 S-bar_i (g_ij +
+// gsum_i), gbar_ji = 2 S-bar_j (-g_ij + gsum_j).
 __global__ void __launch_bounds__(kBlock)
     k_adj_it3(IterArgs a, const pbf_float4* __restrict__ gsum, const float* __restrict__ cterm,
               const float* __restrict__ lamb, pbf_float4* __restrict__ pb) {
@@ -423,26 +402,28 @@ __global__ void __launch_bounds__(kBlock)
     const Sph s = sph_consts(a.h);
     const float gs = a.mass / a.rho0;
     const float4 gsi = ld4(gsum, i);
-    const float den = gsi.w + a.eps;
-    const float lb = lamb[i];
-    const float Cb = -lb / den;
-    const float Sb = lb * cterm[i] / (den * den);
+    const float2 bi = lambda_bars(lamb[i], cterm[i], gsi.w, a.eps);
     const float3 pi = f3(ld4(a.pk, i));
     const float3 gsv = f3(gsi);
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
-    const uint32_t* ids = a.b.ids;
     for_each_neighbour<true>(
-        a.b.nbr, a.b.stride, i, a.b.cnt[i], [&](uint32_t j) { return Nbr2{j, ld4(a.pk, j), {}}; },
-        [&](const Nbr2& rr) {
-            const uint32_t j = rr.j;
-            const float3 d = pi - f3(rr.a);
+        a.b.nbr, a.b.stride, i, a.b.cnt[i],
+        [&](uint32_t j) { return Nbr2{j, ld4(a.pk, j), ld4(gsum, j)}; },
+        [&](const Nbr2& r) {
+            const float2 bj = lambda_bars(lamb[r.j], cterm[r.j], r.b.w, a.eps);
+            const float3 d = pi - f3(r.a);
             const float3 g = (-gs) * spiky(d, s);
-            const float3 gb = (2.0f * Sb) * (g + gsv);
-            const float3 db = (Cb * gs) * poly6_grad(d, s) + (-gs) * spiky_jac(d, gb, s);
-            pacc = pacc + db;
-            atomic_add3(pb, ids[j], make_float3(-db.x, -db.y, -db.z));
+            const float3 gbi = (2.0f * bi.y) * (g + gsv);
+            const float3 gbj = (2.0f * bj.y) * (f3(r.b) - g);
+            pacc = pacc + ((bi.x + bj.x) * gs) * poly6_grad(d, s) +
+                   (-gs) * spiky_jac(d, gbi - gbj, s);
         });
-    atomic_add3(pb, ids[i], pacc);
+    const uint32_t gid = a.b.ids[i];
+    float4 p = ld4(pb, gid);
+    p.x += pacc.x;
+    p.y += pacc.y;
+    p.z += pacc.z;
+    st4(pb, gid, p);
 }
 
 // Integrate: p0 = x + v1 dt, v1 = v + g dt.  In id order.
@@ -488,23 +469,16 @@ pbf_status adjoint_post(const AdjPost& t, const AdjBuffers& w, size_t n, const A
     a.friction = prm.friction;
     a.box = prm.box;
     const unsigned g = grid_for(n);
-    pbf_status st;
-    hipLaunchKernelGGL(k_adj_post1, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.vb, w.v1b, w.v3b, w.etab, w.wb, w.wlb, w.acc);
-    if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
-    hipLaunchKernelGGL(k_adj_post2, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.etab, w.wlb, w.pb);
-    if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
-    hipLaunchKernelGGL(k_adj_post3, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.wlb, w.wb, w.v2b);
-    if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
-    hipLaunchKernelGGL(k_adj_post4, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.wb, w.v3b, w.pb);
-    if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
-    hipLaunchKernelGGL(k_adj_post5, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.v3b, w.v2b, w.pb, w.acc);
-    if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
-    PBF_LAUNCH(k_adj_post6, g, kBlock, stream, a, w.v2b, w.pb, w.x0b);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    pbf_status rc;
+    hipLaunchKernelGGL(k_adj_post1, dim3(g), dim3(kBlock), 0, st, a, w.vb, w.v1b, w.v3b, w.etab,
+                       w.wb, w.acc);
+    if ((rc = pbf_hip_check(hipGetLastError())) != PBF_OK) return rc;
+    hipLaunchKernelGGL(k_adj_post2, dim3(g), dim3(kBlock), 0, st, a, w.etab, w.wb, w.pb);
+    if ((rc = pbf_hip_check(hipGetLastError())) != PBF_OK) return rc;
+    hipLaunchKernelGGL(k_adj_post4, dim3(g), dim3(kBlock), 0, st, a, w.wb, w.v3b, w.pb);
+    if ((rc = pbf_hip_check(hipGetLastError())) != PBF_OK) return rc;
+    PBF_LAUNCH(k_adj_post5, g, kBlock, stream, a, w.v3b, w.pb, w.x0b, w.acc);
 }
 
 pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n, const AdjParams& prm,
@@ -527,7 +501,7 @@ pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n, co
     const unsigned g = grid_for(n);
     pbf_status st;
     hipLaunchKernelGGL(k_adj_it1, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
-                       w.pb, w.prb, w.gsum, w.cterm, w.lamb);
+                       w.pb, w.prb, w.gsum, w.cterm);
     if ((st = pbf_hip_check(hipGetLastError())) != PBF_OK) return st;
     hipLaunchKernelGGL(k_adj_it2, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,
                        w.prb, w.lamb, w.pb);

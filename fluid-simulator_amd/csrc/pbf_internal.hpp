@@ -132,12 +132,10 @@ struct AdjBuffers {
     pbf_float4* x0b;
     pbf_float4* v1b;
     pbf_float4* v3b;
-    pbf_float4* v2b;
     pbf_float4* etab;
     pbf_float4* wb;
     pbf_float4* prb;
     pbf_float4* gsum;
-    float* wlb;
     float* cterm;
     float* lamb;
     double* acc;

@@ -120,10 +120,15 @@ def test_torch_rollout_gradients_equal_the_adjoint(gpu):
     x = s.copyPositionsToHost()
     xbar = np.zeros_like(x)
     xbar[:, :3] = (2.0 * (x[:, :3] - target.numpy()) / sc.n).astype(np.float32)
-    _, v0b, cb, eb = adj.backward(xbar, None)
-    # two runs of the adjoint sum through atomics in different orders: equal
-    # to rounding, not bitwise
-    g = v0b[:, :3].astype(np.float64)
-    assert np.abs(v.grad.numpy() - g).max() <= 1e-4 * np.abs(g).max()
-    assert abs(c.grad.item() - cb) <= 1e-4 * abs(cb)
-    assert abs(e.grad.item() - eb) <= 1e-4 * abs(eb)
+    x0b, v0b, cb, eb = adj.backward(xbar, None)
+    # The position and velocity adjoints are sums in a fixed order (each slot
+    # pulls its pair terms; no atomics), so they are bitwise repeatable; the
+    # two parameter adjoints add per-wave FP64 partials atomically, equal to
+    # FP64 rounding.
+    assert np.array_equal(v.grad.numpy(), v0b[:, :3].astype(np.float64))
+    assert abs(c.grad.item() - cb) <= 1e-12 * abs(cb)
+    assert abs(e.grad.item() - eb) <= 1e-12 * abs(eb)
+    x0b2, v0b2, cb2, eb2 = adj.backward(xbar, None)
+    assert np.array_equal(x0b.view(np.uint32), x0b2.view(np.uint32))
+    assert np.array_equal(v0b.view(np.uint32), v0b2.view(np.uint32))
+    assert abs(cb2 - cb) <= 1e-12 * abs(cb) and abs(eb2 - eb) <= 1e-12 * abs(eb)
