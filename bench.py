@@ -31,8 +31,9 @@ Prints one JSON line:
                  loop of PBFSolver.step() over one solver per scene, and the
                  speedup (--no-batched skips it)
   adjoint        the differentiable workload (65,536 particles, 60 frames,
-                 checkpoints every 10): the adjoint's backward pass against a
-                 plain run of the same frames (--no-adjoint skips it)
+                 checkpoints every 10, and one 60-frame segment taped by
+                 forward): the adjoint's passes against a plain run of the
+                 same frames (--no-adjoint skips it)
   cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on
                  the host cores, on a bounded sample of the same workload
 
@@ -251,10 +252,23 @@ def adjoint_cost(pbf_amd, scenes, frames: int = 60, every: int = 10) -> dict:
     t0 = time.perf_counter()
     adj.backward(xbar, None)
     bwd = time.perf_counter() - t0
+    # One segment: forward() tapes every substep as it runs them, backward()
+    # replays nothing.
+    s.setParticles(sc.positions, sc.velocities)
+    one = pbf_amd.PBFAdjoint(s)
+    t0 = time.perf_counter()
+    one.forward(frames, frames)
+    pbf_amd.synchronize()
+    fwd1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    one.backward(xbar, None)
+    bwd1 = time.perf_counter() - t0
     return {"workload": f"{sc.name}: {sc.n} particles, {frames} frames of "
                         f"{sc.params.substeps} substep(s), checkpoints every {every}",
             "run_ms": plain * 1e3, "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
-            "backward_over_run": bwd / plain}
+            "backward_over_run": bwd / plain,
+            "one_segment": {"forward_ms": fwd1 * 1e3, "backward_ms": bwd1 * 1e3,
+                            "forward_plus_backward_over_run": (fwd1 + bwd1) / plain}}
 
 
 def main() -> int:

@@ -294,7 +294,8 @@ struct PBFSolver::State {
         SubstepTape* tp = tape();
         if (tp == nullptr) return;
         tape_copy(tp->ids[b], id, count);
-        tape_copy(tp->nbr[b], nbr, nbr.size());
+        launch_check(pbfint::tape_lists(nbr.data(), tp->nbr[b].data(), cnt.data(), count, stride,
+                                        kMaxNeighbors / 4, stream));
         tape_copy(tp->cnt[b], cnt, count);
     }
 
@@ -1046,6 +1047,9 @@ struct PBFAdjoint::Impl {
     std::vector<Checkpoint> checkpoints;  // frame 0, every, 2 every, ...
     Checkpoint last;                      // the state forward() ended in
     std::vector<SubstepTape> tapes;
+    // The segment whose substeps the tapes hold (forward() tapes its last
+    // segment as it runs it; backward() ends holding segment 0); npos: none.
+    std::size_t tapedSegment = static_cast<std::size_t>(-1);
     DeviceBuffer<float4> pb, vb, x0b, v1b, v3b, etab, wb, prb, gsum;
     DeviceBuffer<float> cterm, lamb;
     DeviceBuffer<double> acc;
@@ -1091,17 +1095,36 @@ void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
     m.every = checkpointEvery;
     m.n = s.count;
     m.checkpoints.clear();
-    m.tapes.clear();
+    m.tapedSegment = static_cast<std::size_t>(-1);
     s.reset_status();
-    for (std::size_t f = 0; f < frameCount; ++f) {
-        if (f % checkpointEvery == 0) {
-            m.checkpoints.emplace_back();
-            m.save(m.checkpoints.back());
+    // The last segment is taped as it runs: backward() starts with it, and
+    // its replay would record exactly these substeps again.
+    const std::size_t lastSegment = frameCount == 0 ? 0 : (frameCount - 1) / checkpointEvery;
+    const std::size_t lastFirst = lastSegment * checkpointEvery;
+    try {
+        for (std::size_t f = 0; f < frameCount; ++f) {
+            if (f % checkpointEvery == 0) {
+                m.checkpoints.emplace_back();
+                m.save(m.checkpoints.back());
+            }
+            if (f == lastFirst) {
+                const std::size_t subs =
+                    (frameCount - lastFirst) * static_cast<std::size_t>(s.params.substeps);
+                if (m.tapes.size() < subs) m.tapes.resize(subs);
+                for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(m.n, s.nbr.size());
+                s.tapes = &m.tapes;
+                s.tapeNext = 0;
+            }
+            s.frame();
         }
-        s.frame();
+    } catch (...) {
+        s.tapes = nullptr;
+        throw;
     }
+    s.tapes = nullptr;
     m.save(m.last);
     s.finish();
+    if (frameCount != 0) m.tapedSegment = lastSegment;
 }
 
 void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t particleCount,
@@ -1156,21 +1179,25 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
         const std::size_t f0 = k * m.every;
         const std::size_t f1 = std::min(m.frames, f0 + m.every);
         const std::size_t subs = (f1 - f0) * static_cast<std::size_t>(P.substeps);
-        if (m.tapes.size() < subs) m.tapes.resize(subs);
-        for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(n, s.nbr.size());
-        // Replay the segment with the tape on.
-        m.restore(m.checkpoints[k]);
-        s.reset_status();
-        s.tapes = &m.tapes;
-        s.tapeNext = 0;
-        try {
-            for (std::size_t f = f0; f < f1; ++f) s.enqueue_frame();
-        } catch (...) {
+        if (k != m.tapedSegment) {
+            // Replay the segment with the tape on.
+            if (m.tapes.size() < subs) m.tapes.resize(subs);
+            for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(n, s.nbr.size());
+            m.tapedSegment = static_cast<std::size_t>(-1);
+            m.restore(m.checkpoints[k]);
+            s.reset_status();
+            s.tapes = &m.tapes;
+            s.tapeNext = 0;
+            try {
+                for (std::size_t f = f0; f < f1; ++f) s.enqueue_frame();
+            } catch (...) {
+                s.tapes = nullptr;
+                throw;
+            }
             s.tapes = nullptr;
-            throw;
+            s.finish();
+            m.tapedSegment = k;
         }
-        s.tapes = nullptr;
-        s.finish();
         // Back through the segment's substeps.
         for (std::size_t q = subs; q-- > 0;) {
             SubstepTape& tp = m.tapes[q];

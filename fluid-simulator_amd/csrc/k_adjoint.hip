@@ -445,9 +445,27 @@ __global__ void __launch_bounds__(kBlock)
     st4(v1b, i, v);
 }
 
+// The groups of a grouped list that hold entries: ceil(cnt / 4) of each
+// slot (the list kernels read no others), at most maxGroups.
+__global__ void __launch_bounds__(kBlock)
+    k_tape_lists(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                 const int* __restrict__ cnt, size_t n, size_t stride, int maxGroups) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int groups = min((cnt[i] + 3) / 4, maxGroups);
+    for (int g = 0; g < groups; ++g) dst[static_cast<size_t>(g) * stride + i] = src[static_cast<size_t>(g) * stride + i];
+}
+
 }  // namespace
 
 namespace pbfint {
+
+pbf_status tape_lists(const uint32_t* src, uint32_t* dst, const int* cnt, size_t n, size_t stride,
+                      int maxGroups, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_tape_lists, grid_for(n), kBlock, stream, reinterpret_cast<const uint4*>(src),
+               reinterpret_cast<uint4*>(dst), cnt, n, stride, maxGroups);
+}
 
 pbf_status adjoint_post(const AdjPost& t, const AdjBuffers& w, size_t n, const AdjParams& prm,
                         void* stream) {

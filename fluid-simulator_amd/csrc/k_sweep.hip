@@ -3,8 +3,9 @@
 // Builds exactly the list findNeighbors builds (src/pbf/neighbors.cu:3-91):
 // for particle i, every other particle j of the 3x3x3 cells around i's cell
 // with d2 <= h*h (FP32, no FMA), in z, y, x cell order and sorted order
-// within a cell, stored slot-major at nbr[q * stride + i] (at most maxN
-// entries; the count stays whole and raises the overflow bit above maxN).
+// within a cell (at most maxN entries; the count stays whole and raises the
+// overflow bit above maxN).  The reference stores it slot-major at
+// nbr[q * stride + i]; this build stores it in groups of four (below).
 // The reference reads every candidate through global memory per particle;
 // here the candidates are staged in LDS per tile:
 //

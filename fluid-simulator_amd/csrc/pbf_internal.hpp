@@ -155,6 +155,10 @@ pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n,
                              const AdjParams& p, void* stream);
 // Integrate backward: completes x0b and v1b from pb.
 pbf_status adjoint_predict(const AdjBuffers& w, size_t n, float dt, void* stream);
+// Copies the groups of the grouped lists src (stride slots per group row)
+// that hold entries, ceil(cnt[i] / 4) of slot i (at most maxGroups), to dst.
+pbf_status tape_lists(const uint32_t* src, uint32_t* dst, const int* cnt, size_t n, size_t stride,
+                      int maxGroups, void* stream);
 
 // Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
 // have cell size == h (a 3x3x3 stencil) and the working set must be in the

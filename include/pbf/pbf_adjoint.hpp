@@ -2,10 +2,12 @@
 // reference (its solver has no derivatives).
 //
 // forward() runs frames of a solver and keeps its working set every
-// `checkpointEvery` frames; backward() replays each checkpoint segment with
-// a tape (neighbour lists, orderings and per-pass records of every
-// substep) and runs the hand-written adjoint kernels back through it, so
-// memory holds one segment's tape at a time.  Neighbour lists and every
+// `checkpointEvery` frames, and records a tape (neighbour lists, orderings
+// and per-pass records of every substep) of the last segment as it runs it;
+// backward() runs the hand-written adjoint kernels back through that tape,
+// then replays each earlier segment from its checkpoint with the tape on and
+// goes back through it, so memory holds one segment's tape at a time.  With
+// checkpointEvery >= frameCount nothing is replayed.  Neighbour lists and every
 // discrete decision of the forward (container projections, wall contacts)
 // are held fixed; between them the derivatives are exact up to FP32
 // rounding.  Models the container only (no spheres, boxes or planes) and

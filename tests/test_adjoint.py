@@ -83,3 +83,34 @@ def test_substep_adjoint_matches_central_differences(seed, restitution, friction
         hs = 1e-7 if name == "x" else 1e-6
         fd = (fwd(hs) - fwd(-hs)) / (2 * hs)
         assert abs(fd - adj) <= 2e-6 * max(abs(fd), 1e-3), f"{name}: fd {fd} adjoint {adj}"
+
+
+def test_fp64_forward_matches_the_oracle_step():
+    """run_full (the forward whose central differences the GPU adjoint is
+    checked against) is the solver step: two substeps of a 512-particle
+    block with XSPH and vorticity on, against the FP64 oracle (which rounds
+    its state to FP32 between kernels, so agreement is to FP32 rounding)."""
+    from oracle import OracleSolver
+    from pbf_amd import scenes
+
+    sc = scenes.small_block(8, 8, 8, substeps=2)
+    P = sc.params
+    P.xsphViscosity = 1e-4
+    P.vorticityStrength = 1e-4
+    rng = np.random.default_rng(7)
+    v0 = sc.velocities.copy()
+    v0[:, :3] = rng.normal(0.0, 0.1, (sc.n, 3)).astype(np.float32)
+    o = OracleSolver("f64")
+    o.initialize(sc.n, sc.minBounds, sc.maxBounds, P)
+    o.setParticles(sc.positions, v0)
+    o.step()
+    xo, vo = o.copyPositionsToHost()[:, :3], o.copyVelocitiesToHost()[:, :3]
+    Q = A.Params(dt=P.dt / P.substeps, h=P.smoothingRadius, mass=P.particleMass,
+                 rho0=P.restDensity, eps_lambda=P.lambdaEpsilon,
+                 gravity=(P.gravity.x, P.gravity.y, P.gravity.z), scorr_k=P.scorrK,
+                 scorr_n=P.scorrN, scorr_dq=P.scorrDeltaQ, radius=P.particleRadius,
+                 box_min=sc.minBounds, box_max=sc.maxBounds)
+    x, v = A.run_full(sc.positions[:, :3].astype(np.float64), v0[:, :3].astype(np.float64),
+                      P.substeps, Q, P.xsphViscosity, P.vorticityStrength)
+    assert np.abs(x - xo).max() <= 1e-5 * np.abs(xo).max()
+    assert np.linalg.norm(v - vo) <= 1e-4 * np.linalg.norm(vo)

@@ -71,3 +71,26 @@ def test_batched_graph_frames_equal_eager(gpu):
             b.run(2)
         runs.append(b.copyPositionsToHost())
     assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+
+
+def test_batch_raises_the_overflow_of_one_scene(gpu):
+    """A scene that overflows its neighbour capacity alone (the literal
+    headline strengths diverge in the first frame, as in the oracle) makes
+    the batch raise the same error; the other scenes cannot mask it."""
+    import pbf_amd
+
+    bad = scenes.dam_break(1, literal_strengths=True)
+    good = scenes.batch_scene(3, nmin=500, nmax=2000, substeps=1)
+    params = bad.params
+    one = gpu.Solver()
+    one.initialize(bad.n, bad.minBounds, bad.maxBounds, params)
+    one.setParticles(bad.positions, bad.velocities)
+    with pytest.raises(pbf_amd.NeighborOverflow):
+        one.step()
+    b = pbf_amd.PBFBatchSolver()
+    b.initialize([(good.n, good.minBounds, good.maxBounds), (bad.n, bad.minBounds, bad.maxBounds)],
+                 params)
+    b.setParticles(np.concatenate([good.positions, bad.positions]),
+                   np.concatenate([good.velocities, bad.velocities]))
+    with pytest.raises(pbf_amd.NeighborOverflow):
+        b.step()
