@@ -7,13 +7,19 @@ resident in HBM before the timed region.  The throughput unit is the
 particle-step = one particle advanced by one PBF substep; --steps counts
 PBFSolver::step() calls (frames).
 
-Two passes over the same frames from the same seeded state (the step is
+After the warm-up frames the library's autotuner times every launch
+tuning (the skin of the neighbour lists, i.e. the grid's cell-size
+multiplier, by the neighbour passes' workgroup size) on a few frames from
+the warmed-up state, restores that state and keeps the fastest.  Two passes
+over the same frames from the same seeded state and tuning (the step is
 bitwise deterministic): the first is timed end to end for `value`; the
 second records HIP events around every launch on the solver's stream, which
 gives each kernel's average launch duration over exactly the timed frames.
 
 Prints one JSON line:
   value          particle-steps/s over exactly --steps timed steps
+  tuning         the autotuner's candidates, their milliseconds per frame,
+                 the choice, and the stale frames of the timed pass
   roofline       dominant kernel (the neighbour-list build): algorithmic
                  HBM bytes per launch (bytes per particle of its phase x
                  particles) / its average launch duration, against 8 TB/s;
@@ -167,12 +173,26 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
-def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool):
+def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=None,
+               tune_frames: int = 0):
+    """Warm-up frames, then (tune_frames > 0) the library's autotuner from
+    the warmed-up state (it restores that state), or the given tuning, then
+    `steps` timed frames.  Returns (seconds, kernel times, tuning report)."""
     solver = pbf_amd.PBFSolver()
     solver.initialize(scene.n, scene.minBounds, scene.maxBounds, scene.params)
     solver.setParticles(scene.positions, scene.velocities)
     solver.run(warmup)
     pbf_amd.synchronize()
+    report = None
+    if tune_frames > 0:
+        chosen, table = solver.autotune(tune_frames)
+        report = {"chosen": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock},
+                  "frames_per_candidate": tune_frames,
+                  "candidates": [{"neighbor_skin": a, "pass_block": b, "ms_per_frame": ms}
+                                 for a, b, ms in table]}
+    elif tuning is not None:
+        solver.setTuning(tuning["neighbor_skin"], tuning["pass_block"])
+    stale0 = solver.staleFrames()
     if profile:
         solver.setProfiling(True)
     t0 = time.perf_counter()
@@ -180,7 +200,9 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool):
     pbf_amd.synchronize()
     elapsed = time.perf_counter() - t0
     times = solver.kernelTimes() if profile else {}
-    return elapsed, times
+    if report is not None:
+        report["stale_frames_timed"] = solver.staleFrames() - stale0
+    return elapsed, times, report
 
 
 def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2) -> dict:
@@ -199,15 +221,18 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
                    np.concatenate([s.velocities for s in scs]))
     b.run(warmup)
     pbf_amd.synchronize()
+    chosen, _ = b.autotune(1)
     t0 = time.perf_counter()
     b.run(frames)
     pbf_amd.synchronize()
     tb = (time.perf_counter() - t0) / frames
+    stale = b.staleFrames()
     del b
     solvers = []
     for s in scs:
         o = pbf_amd.PBFSolver()
         o.initialize(s.n, s.minBounds, s.maxBounds, s.params)
+        o.setTuning(chosen.neighborSkin, chosen.passBlock)
         o.setParticles(s.positions, s.velocities)
         o.run(warmup)
         solvers.append(o)
@@ -218,6 +243,8 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
             o.step()
     tl = (time.perf_counter() - t0) / frames
     return {"scenes": count, "particles": n, "substeps": sub, "frames": frames,
+            "tuning": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock},
+            "stale_frames": stale,
             "batched_value": n * sub / tb, "looped_value": n * sub / tl,
             "unit": "particle-steps/s", "batched_ms_per_step": tb * 1e3,
             "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
@@ -285,6 +312,10 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (65,536 particles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default tuning (the reference's builds, 256-thread passes)")
+    ap.add_argument("--tune-frames", type=int, default=3,
+                    help="frames the autotuner times per candidate")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -305,7 +336,8 @@ def main() -> int:
     # One timed "step" is one PBFSolver::step() call, i.e. one frame of
     # `substeps` PBF substeps; throughput counts particle-substeps (the
     # reference benchmark's particle-substeps/s, pbf_benchmark.cu:201-211).
-    elapsed, _ = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=False)
+    elapsed, _, tuning = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=False,
+                                    tune_frames=0 if a.no_autotune else a.tune_frames)
     value = n * a.steps * substeps / elapsed
 
     grid = pbf_amd.SpatialGrid()
@@ -316,7 +348,8 @@ def main() -> int:
     kernel_rows = {}
     roofline = valu = None
     if not a.no_profile_pass:
-        _, times = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=True)
+        _, times, _ = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=True,
+                                 tuning=tuning["chosen"] if tuning else None)
         for name, (ms, launches) in times.items():
             avg = ms / max(launches, 1)
             b = alg_bytes(name, n, key_bits, cells)
@@ -387,6 +420,7 @@ def main() -> int:
                                "XSPH + vorticity + s_corr + container",
                    "particles": n, "iterations": scene.params.solverIterations,
                    "substeps": substeps, "parallelism": "single-gpu"},
+        "tuning": tuning,
         "roofline": roofline,
         "valu": valu,
         "step_roofline": step_roof,

@@ -94,6 +94,33 @@ void write_kernel_times(const std::vector<pbf::PBFSolver::KernelTime>& times, ch
 
 }  // namespace
 
+namespace {
+pbf::PBFSolver::Tuning from_c(const pbf_tuning& t) {
+    return pbf::PBFSolver::Tuning{t.neighborSkin, t.passBlock};
+}
+pbf_tuning to_c(const pbf::PBFSolver::Tuning& t) { return pbf_tuning{t.neighborSkin, t.passBlock}; }
+
+template <class Tuner>
+pbf_status run_autotune(Tuner&& tune, const pbf_tuning* candidates, size_t count,
+                        double* msPerFrame, pbf_tuning* chosen) {
+    if (candidates == nullptr && count != 0)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "candidates must not be null");
+    return guarded([&] {
+        std::vector<pbf::PBFSolver::Tuning> c;
+        for (size_t k = 0; k < count; ++k) c.push_back(from_c(candidates[k]));
+        const std::vector<pbf::PBFSolver::TuningTime> times = tune(c);
+        if (msPerFrame != nullptr)
+            for (size_t k = 0; k < times.size(); ++k) msPerFrame[k] = times[k].msPerFrame;
+        if (chosen != nullptr) {
+            const pbf::PBFSolver::TuningTime* best = nullptr;
+            for (const auto& t : times)
+                if (best == nullptr || t.msPerFrame < best->msPerFrame) best = &t;
+            if (best != nullptr) *chosen = to_c(best->tuning);
+        }
+    });
+}
+}  // namespace
+
 extern "C" {
 
 // ---------------------------------------------------------------- grid
@@ -283,6 +310,40 @@ pbf_status pbf_solver_set_graphs(pbf_solver* s, int enabled) {
     return guarded([&] { s->solver.setGraphs(enabled != 0); });
 }
 
+pbf_status pbf_solver_set_neighbor_skin(pbf_solver* s, float skin) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setNeighborSkin(skin); });
+}
+
+pbf_status pbf_solver_neighbor_skin(const pbf_solver* s, float* skin, size_t* staleFrames) {
+    PBF_REQUIRE(s);
+    if (skin) *skin = s->solver.neighborSkin();
+    if (staleFrames) *staleFrames = s->solver.staleFrames();
+    return PBF_OK;
+}
+
+
+pbf_status pbf_solver_set_tuning(pbf_solver* s, const pbf_tuning* t) {
+    PBF_REQUIRE(s);
+    PBF_REQUIRE(t);
+    return guarded([&] { s->solver.setTuning(from_c(*t)); });
+}
+
+pbf_status pbf_solver_get_tuning(const pbf_solver* s, pbf_tuning* t) {
+    PBF_REQUIRE(s);
+    PBF_REQUIRE(t);
+    *t = to_c(s->solver.tuning());
+    return PBF_OK;
+}
+
+pbf_status pbf_solver_autotune(pbf_solver* s, size_t frames, const pbf_tuning* candidates,
+                               size_t count, double* msPerFrame, pbf_tuning* chosen) {
+    PBF_REQUIRE(s);
+    return run_autotune(
+        [&](const std::vector<pbf::PBFSolver::Tuning>& c) { return s->solver.autotune(frames, c); },
+        candidates, count, msPerFrame, chosen);
+}
+
 pbf_status pbf_solver_set_profiling(pbf_solver* s, int enabled) {
     PBF_REQUIRE(s);
     return guarded([&] { s->solver.setProfiling(enabled != 0); });
@@ -365,6 +426,31 @@ pbf_status pbf_batch_counts(const pbf_batch* b, size_t* sceneCount, size_t* part
 pbf_status pbf_batch_set_graphs(pbf_batch* b, int enabled) {
     PBF_REQUIRE(b);
     return guarded([&] { b->batch.setGraphs(enabled != 0); });
+}
+
+pbf_status pbf_batch_set_neighbor_skin(pbf_batch* b, float skin) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setNeighborSkin(skin); });
+}
+
+pbf_status pbf_batch_stale_frames(const pbf_batch* b, size_t* staleFrames) {
+    PBF_REQUIRE(b);
+    if (staleFrames) *staleFrames = b->batch.staleFrames();
+    return PBF_OK;
+}
+
+pbf_status pbf_batch_set_tuning(pbf_batch* b, const pbf_tuning* t) {
+    PBF_REQUIRE(b);
+    PBF_REQUIRE(t);
+    return guarded([&] { b->batch.setTuning(from_c(*t)); });
+}
+
+pbf_status pbf_batch_autotune(pbf_batch* b, size_t frames, const pbf_tuning* candidates,
+                              size_t count, double* msPerFrame, pbf_tuning* chosen) {
+    PBF_REQUIRE(b);
+    return run_autotune(
+        [&](const std::vector<pbf::PBFSolver::Tuning>& c) { return b->batch.autotune(frames, c); },
+        candidates, count, msPerFrame, chosen);
 }
 
 pbf_status pbf_batch_set_profiling(pbf_batch* b, int enabled) {

@@ -39,6 +39,7 @@ GridCore GridCore::create(std::size_t maxParticles, float3 minBounds, float3 max
     next.keyBits = pbfint::radix_key_bits(next.numCells);
 
     next.keys.allocate(maxParticles);
+    next.keysTmp.allocate(maxParticles);
     next.indices.allocate(maxParticles);
     next.sortedKeys.allocate(maxParticles);
     next.sortedIndices.allocate(maxParticles);
@@ -81,6 +82,7 @@ GridCore GridCore::createKeyed(std::size_t maxParticles, std::size_t numCells, f
     next.numCells = numCells;
     next.keyBits = pbfint::radix_key_bits(numCells);
     next.keys.allocate(maxParticles);
+    next.keysTmp.allocate(maxParticles);
     next.indices.allocate(maxParticles);
     next.sortedKeys.allocate(maxParticles);
     next.sortedIndices.allocate(maxParticles);
@@ -92,7 +94,7 @@ GridCore GridCore::createKeyed(std::size_t maxParticles, std::size_t numCells, f
 
 void GridCore::sortAndRanges(std::size_t n, hipStream_t stream) {
     launch_check(pbfint::radix_sort_pairs(keys.data(), nullptr, sortedKeys.data(),
-                                          sortedIndices.data(), keys.data(), indices.data(), n,
+                                          sortedIndices.data(), keysTmp.data(), indices.data(), n,
                                           keyBits, scratch.data(), stream));
     launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
     launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,

@@ -31,7 +31,8 @@ struct GridCore {
     float cellSize = 0.0f;
     int keyBits = 0;
 
-    DeviceBuffer<std::uint32_t> keys;           // unsorted keys (also a sort ping-pong buffer)
+    DeviceBuffer<std::uint32_t> keys;           // unsorted keys
+    DeviceBuffer<std::uint32_t> keysTmp;        // sort ping-pong buffer for keys
     DeviceBuffer<std::uint32_t> indices;        // sort ping-pong buffer for values
     DeviceBuffer<std::uint32_t> sortedKeys;
     DeviceBuffer<std::uint32_t> sortedIndices;

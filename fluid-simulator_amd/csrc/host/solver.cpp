@@ -25,6 +25,7 @@
 // and the host reads the word once when the step (or run) finishes.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -237,25 +238,59 @@ struct PBFSolver::State {
     GridCore grid;
     CollisionSystem collision;
 
-    DeviceBuffer<float4> x, xa, p, pa, v, va, vin, vina, omega;
+    // pq holds a skin substep's predicted positions (the lists' reference
+    // points) while the iterations ping-pong p and pa.
+    DeviceBuffer<float4> x, xa, p, pa, pq, v, va, vin, vina, omega;
+    // status[0]: the status word; status[1]: frames finished before a
+    // stale one.
     DeviceBuffer<std::uint32_t> id, ida, status;
-    DeviceBuffer<std::uint32_t> nbr;      // slot-major neighbour lists of the last build
+    DeviceBuffer<std::uint32_t> nbr;      // grouped neighbour lists of the last build
     DeviceBuffer<int> cnt;
     std::size_t stride = 0;
     DeviceBuffer<std::uint32_t> tileEnd;  // sweep tile ends of the last build
+
+    // Skin lists: skin > 0 builds the grid once per substep, on cells of
+    // h * (1 + skin) (sgrid); a frame whose lists went stale is run again
+    // from the checkpoint with a build per iteration.
+    float skin = 0.0f;
+    GridCore sgrid;
+    DeviceBuffer<std::uint32_t> stileEnd;
+    DeviceBuffer<pbfint::SceneGeom> sceneGeomSkin;
+    DeviceBuffer<float4> ckx, ckv;
+    DeviceBuffer<std::uint32_t> ckid;
+    bool forceExact = false;
+    std::size_t staleFrames = 0;
 
     // Scenes of a batched solver (PBFBatchSolver); none for one scene.  Each
     // scene keeps its grid, container and slot range in device tables.
     std::size_t sceneCount = 0;
     std::vector<std::size_t> sceneFirst;  // sceneCount + 1 slot offsets
+    std::vector<pbf_container> sceneBoxes;
     DeviceBuffer<pbfint::SceneGeom> sceneGeom;
     DeviceBuffer<pbf_container> sceneContainers;
     DeviceBuffer<std::uint32_t> sceneOf;
 
-    pbfint::SceneTable scenes() const {
+    pbfint::SceneTable scenes(bool sk = false) const {
         if (sceneCount == 0) return pbfint::SceneTable{};
-        return pbfint::SceneTable{sceneGeom.data(), sceneContainers.data(), sceneOf.data()};
+        return pbfint::SceneTable{sk ? sceneGeomSkin.data() : sceneGeom.data(),
+                                  sceneContainers.data(), sceneOf.data()};
     }
+    GridCore& grid_of(bool sk) { return sk ? sgrid : grid; }
+    std::uint32_t* tiles_of(bool sk) { return sk ? stileEnd.data() : tileEnd.data(); }
+    // Workgroup size of the neighbour passes (a tuning: results do not
+    // depend on it).
+    int passBlock = 256;
+    pbfint::ListPass lpass() const {
+        pbfint::ListPass lp;
+        lp.block = passBlock;
+        return lp;
+    }
+    bool skin_active() const {
+        return skin > 0.0f && tapes == nullptr && !forceExact && !adjointExact;
+    }
+    bool adjointExact = false;  // the adjoint differentiates the reference dataflow
+    // (Re)builds the skin grid for the current skin, bounds and scenes.
+    void make_skin_grid(std::size_t capacity);
 
     void allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles);
 
@@ -270,10 +305,12 @@ struct PBFSolver::State {
     // with the permutation the frame applies; a workload cycles through one
     // or two of them.  Kernel arguments are baked in too: initialize,
     // setParticles and the collider setters drop the graphs.
-    using BufferKey = std::array<const void*, 10>;
+    static constexpr int kSlots = 11;
+    using BufferKey = std::array<const void*, kSlots>;
     struct FrameGraph {
         BufferKey start{};
-        std::array<int, 10> perm{};  // slot k ends up holding slot perm[k]'s buffer
+        bool skin = false;               // a frame with skin lists
+        std::array<int, kSlots> perm{};  // slot k ends up holding slot perm[k]'s buffer
         hipGraphExec_t exec = nullptr;
     };
     std::vector<FrameGraph> graphs;
@@ -309,28 +346,29 @@ struct PBFSolver::State {
         graphs.clear();
     }
 
-    // Slots 0-7: x, xa, p, pa, v, va, vin, vina; 8-9: id, ida.
+    // Slots 0-8: x, xa, p, pa, pq, v, va, vin, vina; 9-10: id, ida.
+    static constexpr int kF4Slots = 9;
     BufferKey buffer_key() const {
-        return {x.data(), xa.data(), p.data(), pa.data(), v.data(), va.data(),
+        return {x.data(), xa.data(), p.data(), pa.data(), pq.data(), v.data(), va.data(),
                 vin.data(), vina.data(), id.data(), ida.data()};
     }
 
-    void permute(const std::array<int, 10>& perm) {
-        std::array<DeviceBuffer<float4>*, 8> f{&x, &xa, &p, &pa, &v, &va, &vin, &vina};
+    void permute(const std::array<int, kSlots>& perm) {
+        std::array<DeviceBuffer<float4>*, kF4Slots> f{&x, &xa, &p, &pa, &pq, &v, &va, &vin, &vina};
         std::array<DeviceBuffer<std::uint32_t>*, 2> u{&id, &ida};
-        std::array<DeviceBuffer<float4>, 8> tf;
+        std::array<DeviceBuffer<float4>, kF4Slots> tf;
         std::array<DeviceBuffer<std::uint32_t>, 2> tu;
-        for (int k = 0; k < 8; ++k) tf[k] = std::move(*f[perm[k]]);
-        for (int k = 0; k < 2; ++k) tu[k] = std::move(*u[perm[8 + k] - 8]);
-        for (int k = 0; k < 8; ++k) *f[k] = std::move(tf[k]);
+        for (int k = 0; k < kF4Slots; ++k) tf[k] = std::move(*f[perm[k]]);
+        for (int k = 0; k < 2; ++k) tu[k] = std::move(*u[perm[kF4Slots + k] - kF4Slots]);
+        for (int k = 0; k < kF4Slots; ++k) *f[k] = std::move(tf[k]);
         for (int k = 0; k < 2; ++k) *u[k] = std::move(tu[k]);
     }
 
     // The permutation that takes assignment `from` to assignment `to`.
-    static std::array<int, 10> perm_between(const BufferKey& from, const BufferKey& to) {
-        std::array<int, 10> perm{};
-        for (int k = 0; k < 10; ++k) {
-            const int lo = k < 8 ? 0 : 8, hi = k < 8 ? 8 : 10;
+    static std::array<int, kSlots> perm_between(const BufferKey& from, const BufferKey& to) {
+        std::array<int, kSlots> perm{};
+        for (int k = 0; k < kSlots; ++k) {
+            const int lo = k < kF4Slots ? 0 : kF4Slots, hi = k < kF4Slots ? kF4Slots : kSlots;
             perm[k] = -1;
             for (int j = lo; j < hi; ++j)
                 if (from[j] == to[k]) perm[k] = j;
@@ -343,13 +381,14 @@ struct PBFSolver::State {
     // assignment, captured now if there is none; launched eagerly when
     // graphs are off or the step is being profiled (events per launch).
     void frame() {
-        if (!useGraphs || prof.enabled || tapes != nullptr) {
+        if (!useGraphs || prof.enabled || tapes != nullptr || forceExact) {
             enqueue_frame();
             return;
         }
         const BufferKey key = buffer_key();
+        const bool sk = skin_active();
         for (const FrameGraph& g : graphs) {
-            if (g.start == key) {
+            if (g.start == key && g.skin == sk) {
                 hip_check(hipGraphLaunch(g.exec, stream));
                 permute(g.perm);
                 return;
@@ -357,6 +396,7 @@ struct PBFSolver::State {
         }
         FrameGraph fg;
         fg.start = key;
+        fg.skin = sk;
         hipGraph_t graph = nullptr;
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -390,8 +430,9 @@ struct PBFSolver::State {
     // into cell order (SpatialGrid::build + reorderWorkingSet,
     // pbf_solver.cu:154-172, spatial_grid.cu:139-226).  `keysReady` skips
     // the key pass when the predict kernel already wrote the keys.
-    void build_and_reorder(bool keysReady) {
+    void build_and_reorder(bool keysReady, bool sk = false) {
         const std::size_t n = count;
+        GridCore& grid = grid_of(sk);
         if (!keysReady) {
             Scope sc(prof, "keys", stream);
             launch_check(pbfint::launch_keys(p.data(), grid.keys.data(), n, grid.minBounds,
@@ -401,7 +442,7 @@ struct PBFSolver::State {
             Scope sc(prof, "radix_sort", stream);
             launch_check(pbfint::radix_sort_pairs(grid.keys.data(), nullptr,
                                                   grid.sortedKeys.data(),
-                                                  grid.sortedIndices.data(), grid.keys.data(),
+                                                  grid.sortedIndices.data(), grid.keysTmp.data(),
                                                   grid.indices.data(), n, grid.keyBits,
                                                   grid.scratch.data(), stream));
         }
@@ -411,7 +452,7 @@ struct PBFSolver::State {
                                                    grid.numCells, stream));
             launch_check(pbfint::launch_cell_ranges(grid.sortedKeys.data(), grid.cellStart.data(),
                                                     grid.cellEnd.data(), n, grid.dims,
-                                                    tileEnd.data(), scenes(), stream));
+                                                    tiles_of(sk), scenes(sk), stream));
         }
         {
             Scope sc(prof, "gather", stream);
@@ -427,21 +468,143 @@ struct PBFSolver::State {
         std::swap(id, ida);
     }
 
-    pbfint::SweepGrid sweep_grid() {
-        return pbfint::SweepGrid{grid.cellStart.data(), grid.cellEnd.data(),
-                                 grid.sortedKeys.data(), tileEnd.data(), grid.dims, count,
-                                 params.smoothingRadius, kMaxNeighbors, scenes()};
+    pbfint::SweepGrid sweep_grid(bool sk = false) {
+        GridCore& grid = grid_of(sk);
+        pbfint::SweepGrid g{grid.cellStart.data(), grid.cellEnd.data(), grid.sortedKeys.data(),
+                            tiles_of(sk), grid.dims, count,
+                            sk ? grid.cellSize : params.smoothingRadius, kMaxNeighbors,
+                            scenes(sk)};
+        if (sk) {
+            g.skin = true;
+            g.box = collision.state().container.data();
+            g.radius = params.particleRadius;
+        }
+        return g;
     }
 
     // Neighbour lists by cell sweep, then the flat list kernels.
-    void build_lists() {
+    void build_lists(bool sk = false) {
         Scope sc(prof, "build_lists", stream);
-        launch_check(pbfint::sweep_build_lists(sweep_grid(), p.data(), nbr.data(), cnt.data(),
+        launch_check(pbfint::sweep_build_lists(sweep_grid(sk), p.data(), nbr.data(), cnt.data(),
                                                stride, status.data(), stream));
     }
 
     // One frame: params.substeps substeps (pbf_solver.cu:186-325).
     void enqueue_frame() {
+        if (skin_active())
+            enqueue_frame_skin();
+        else
+            enqueue_frame_exact();
+    }
+
+    // The frame with skin lists: per substep one grid build, around the
+    // container points of the predicted positions, with radius
+    // h * (1 + skin).  The passes run the reference's sums over the skin
+    // lists: every entry is tested against h, and an entry beyond h adds
+    // exact zeros (W, grad W and s_corr vanish there), so a pass gives the
+    // sums of the exact list in the skin list's order.  This is the
+    // reference's result as long as each list holds every pair within h at
+    // the positions of each pass, which holds while no particle has moved
+    // more than 0.45 * skin * h from its reference point (two such
+    // particles within h of each other were within h + 0.9 * skin * h of
+    // each other when the list was built).  Each Delta-p pass checks that;
+    // a particle past it raises PBF_STATUS_LIST_STALE, the frame stops, and
+    // the host runs it again from the checkpoint taken at its start, with a
+    // grid build per iteration (finish()).  The counts within h stand in
+    // for the counts of the reference's builds: lambda checks those of the
+    // builds before each iteration, the first post-solve pass that of the
+    // post-solve rebuild.
+    void enqueue_frame_skin() {
+        const SimulationParams& P = params;
+        const std::size_t n = count;
+        const float sdt = P.dt / static_cast<float>(P.substeps);
+        const float invDt = 1.0f / sdt;
+        const float h = P.smoothingRadius;
+        const pbfdev::Colliders col = collision.state().view();
+        const float half = 0.45f * skin * h;
+        pbfint::ListPass sk = lpass();
+        sk.stale2 = half * half;
+        sk.exactMax = kMaxNeighbors;
+        {
+            Scope sc(prof, "checkpoint", stream);
+            launch_check(pbfint::launch_checkpoint(x.data(), v.data(), id.data(), ckx.data(),
+                                                   ckv.data(), ckid.data(), n, status.data(),
+                                                   stream));
+        }
+        for (int substep = 0; substep < P.substeps; ++substep) {
+            {
+                Scope sc(prof, "predict_key", stream);
+                launch_check(pbfint::launch_predict_key(
+                    x.data(), p.data(), v.data(), vin.data(), sgrid.keys.data(),
+                    static_cast<int>(n), sdt, P.gravity, sgrid.minBounds, sgrid.cellSize,
+                    sgrid.dims, col.container, P.particleRadius, scenes(true), status.data(),
+                    stream));
+            }
+            build_and_reorder(true, true);
+            build_lists(true);
+            for (int it = 0; it < P.solverIterations; ++it) {
+                sk.p0 = it == 0 ? p.data() : pq.data();
+                {
+                    Scope sc(prof, "lambda", stream);
+                    launch_check(pbfint::launch_lambda_guarded(
+                        p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
+                        P.restDensity, P.lambdaEpsilon, stride, sk, status.data(), stream));
+                }
+                {
+                    Scope sc(prof, "delta_apply_collide", stream);
+                    launch_check(pbfint::launch_delta_apply_collide(
+                        p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
+                        n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
+                        stride, col, P.particleRadius, scenes(), sk, status.data(), stream));
+                }
+                std::swap(p, pa);
+                if (it == 0) std::swap(pa, pq);  // pq keeps the predicted positions
+            }
+            const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
+            pbfint::ListPass postSkin = lpass();
+            postSkin.exactMax = post ? kMaxNeighbors : 0;
+            if (P.xsphViscosity != 0.0f) {
+                Scope sc(prof, "update_xsph", stream);
+                launch_check(pbfint::launch_update_xsph(
+                    x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
+                    n, h, P.xsphViscosity, invDt, stride, postSkin, status.data(), stream));
+                std::swap(v, va);
+                postSkin.exactMax = 0;
+            } else {
+                Scope sc(prof, "update_velocity", stream);
+                launch_check(pbfint::launch_update_velocity_only(x.data(), p.data(), v.data(), n,
+                                                                 invDt, status.data(), stream));
+            }
+            std::swap(x, p);
+            if (P.vorticityStrength != 0.0f) {
+                {
+                    Scope sc(prof, "vorticity", stream);
+                    launch_check(pbfint::launch_vorticity(
+                        x.data(), v.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h,
+                        omega.data(), p.data(), stride, postSkin, status.data(), stream));
+                }
+                {
+                    Scope sc(prof, "confine_resolve", stream);
+                    launch_check(pbfint::launch_confine_resolve(
+                        x.data(), p.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(),
+                        v.data(), vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
+                        P.particleRadius, P.collisionRestitution, P.collisionFriction, scenes(),
+                        lpass(), status.data(), stream));
+                }
+                std::swap(v, va);
+            } else {
+                Scope sc(prof, "resolve_velocities", stream);
+                launch_check(pbfint::launch_resolve_guarded(
+                    x.data(), vin.data(), v.data(), n, col, P.particleRadius,
+                    P.collisionRestitution, P.collisionFriction, scenes(), status.data(), stream));
+            }
+        }
+        launch_check(pbfint::launch_frame_done(status.data(), stream));
+    }
+
+    // One frame with the reference's dataflow: a grid build before every
+    // constraint iteration and after the solve.
+    void enqueue_frame_exact() {
         const SimulationParams& P = params;
         const std::size_t n = count;
         const float sdt = P.dt / static_cast<float>(P.substeps);
@@ -454,7 +617,7 @@ struct PBFSolver::State {
                 launch_check(pbfint::launch_predict_key(
                     x.data(), p.data(), v.data(), vin.data(), grid.keys.data(),
                     static_cast<int>(n), sdt, P.gravity, grid.minBounds, grid.cellSize, grid.dims,
-                    scenes(), status.data(), stream));
+                    nullptr, 0.0f, scenes(), status.data(), stream));
             }
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder(it == 0);
@@ -464,7 +627,8 @@ struct PBFSolver::State {
                     Scope sc(prof, "lambda", stream);
                     launch_check(pbfint::launch_lambda_guarded(
                         p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
-                        P.restDensity, P.lambdaEpsilon, stride, status.data(), stream));
+                        P.restDensity, P.lambdaEpsilon, stride, lpass(), status.data(),
+                        stream));
                 }
                 if (SubstepTape* tp = tape()) tape_copy(tp->pk[it], p, n);
                 {
@@ -472,7 +636,8 @@ struct PBFSolver::State {
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
-                        stride, col, P.particleRadius, scenes(), status.data(), stream));
+                        stride, col, P.particleRadius, scenes(), lpass(), status.data(),
+                        stream));
                 }
                 if (SubstepTape* tp = tape()) tape_copy(tp->pout[it], pa, n);
                 std::swap(p, pa);
@@ -492,7 +657,7 @@ struct PBFSolver::State {
                 Scope sc(prof, "update_xsph", stream);
                 launch_check(pbfint::launch_update_xsph(
                     x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
-                    n, h, P.xsphViscosity, invDt, stride, status.data(), stream));
+                    n, h, P.xsphViscosity, invDt, stride, lpass(), status.data(), stream));
                 std::swap(v, va);
             } else {
                 Scope sc(prof, "update_velocity", stream);
@@ -506,9 +671,9 @@ struct PBFSolver::State {
                     Scope sc(prof, "vorticity", stream);
                     // p holds the previous x after the swap and is free until
                     // the next substep's prediction: it takes the records.
-                    launch_check(pbfint::launch_vorticity(x.data(), v.data(), nbr.data(),
-                                                          cnt.data(), kMaxNeighbors, n, h,
-                                                          omega.data(), p.data(), stride, stream));
+                    launch_check(pbfint::launch_vorticity(
+                        x.data(), v.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h,
+                        omega.data(), p.data(), stride, lpass(), status.data(), stream));
                 }
                 if (SubstepTape* tp = tape()) tape_copy(tp->omega, omega, n);
                 {
@@ -517,7 +682,7 @@ struct PBFSolver::State {
                         x.data(), p.data(), nbr.data(), cnt.data(), kMaxNeighbors, omega.data(),
                         v.data(), vin.data(), va.data(), n, h, sdt, P.vorticityStrength, stride, col,
                         P.particleRadius, P.collisionRestitution, P.collisionFriction, scenes(),
-                        status.data(), stream));
+                        lpass(), status.data(), stream));
                 }
                 std::swap(v, va);
             } else {
@@ -530,7 +695,7 @@ struct PBFSolver::State {
         }
     }
 
-    void reset_status() { launch_check(pbfint::launch_zero_u32(status.data(), 1, stream)); }
+    void reset_status() { launch_check(pbfint::launch_zero_u32(status.data(), 2, stream)); }
 
     // Host -> device copy of n records, each record checked first: one DMA
     // from pinned sources, else through the pinned stage (the check and the
@@ -600,15 +765,50 @@ struct PBFSolver::State {
     }
 
     // Waits for the device and raises what the status word recorded.
-    void finish() {
+    // Returns the status word.
+    std::uint32_t finish() {
         hip_check(hipStreamSynchronize(stream));
         prof.collect();
-        std::uint32_t st = 0;
-        hip_check(hipMemcpy(&st, status.data(), sizeof(st), hipMemcpyDeviceToHost));
-        if (st & PBF_STATUS_NEIGHBOR_OVERFLOW)
+        std::uint32_t st[2] = {0, 0};
+        hip_check(hipMemcpy(st, status.data(), sizeof(st), hipMemcpyDeviceToHost));
+        if (st[0] & PBF_STATUS_NEIGHBOR_OVERFLOW)
             throw std::overflow_error("Particle neighbor count exceeds solver capacity");
-        if (st & PBF_STATUS_COLLISION_NONCONVERGENCE)
+        if (st[0] & PBF_STATUS_COLLISION_NONCONVERGENCE)
             throw collision_nonconvergence("Collision constraints did not converge");
+        framesDone = st[1];
+        return st[0];
+    }
+    std::uint32_t framesDone = 0;
+
+    // frameCount frames, each a step(): a frame whose skin lists went stale
+    // is run again from its checkpoint with a build per iteration, then the
+    // rest follow.  Overflow and non-convergence throw as in the reference.
+    void run_frames(std::size_t frameCount) {
+        while (frameCount > 0) {
+            reset_status();
+            for (std::size_t f = 0; f < frameCount; ++f) frame();
+            if ((finish() & PBF_STATUS_LIST_STALE) == 0u) return;
+            const std::size_t done = framesDone;
+            const std::size_t n = count;
+            hip_check(hipMemcpyAsync(x.data(), ckx.data(), n * sizeof(float4),
+                                     hipMemcpyDeviceToDevice, stream));
+            hip_check(hipMemcpyAsync(v.data(), ckv.data(), n * sizeof(float4),
+                                     hipMemcpyDeviceToDevice, stream));
+            hip_check(hipMemcpyAsync(id.data(), ckid.data(), n * sizeof(std::uint32_t),
+                                     hipMemcpyDeviceToDevice, stream));
+            ++staleFrames;
+            reset_status();
+            forceExact = true;
+            try {
+                frame();
+            } catch (...) {
+                forceExact = false;
+                throw;
+            }
+            forceExact = false;
+            finish();
+            frameCount -= done + 1;
+        }
     }
 
     void copy_out(const DeviceBuffer<float4>& src, DeviceBuffer<float4>& scratch, float4* out,
@@ -687,16 +887,77 @@ void check_container(float3 minBounds, float3 maxBounds, const SimulationParams&
         throw std::invalid_argument("Particle diameter exceeds closed container dimensions");
 }
 
+// The grids of a batch's scenes (their bounds, cells of size cs) laid end
+// to end: global cell and sweep-tile offsets, and slot ranges.
+std::vector<pbfint::SceneGeom> scene_grids(const std::vector<pbf_container>& boxes,
+                                           const std::vector<std::size_t>& first, float cs,
+                                           std::size_t* cellsOut, std::size_t* tilesOut) {
+    std::vector<pbfint::SceneGeom> geom(boxes.size());
+    std::size_t cells = 0, tiles = 0;
+    for (std::size_t k = 0; k < boxes.size(); ++k) {
+        std::size_t sceneCells = 0;
+        const int3 d = GridCore::dimsFor(boxes[k].min, boxes[k].max, cs, &sceneCells);
+        const std::size_t sceneTiles = pbfint::sweep_tile_count(d);
+        if (cells + sceneCells > std::numeric_limits<std::uint32_t>::max() ||
+            tiles + sceneTiles > std::numeric_limits<std::uint32_t>::max())
+            throw std::length_error("Batched grids exceed 32-bit key capacity");
+        pbfint::SceneGeom& g = geom[k];
+        g.minBounds = boxes[k].min;
+        g.dims = d;
+        g.cellOff = static_cast<std::uint32_t>(cells);
+        g.tileOff = static_cast<std::uint32_t>(tiles);
+        g.chunks = static_cast<std::uint32_t>((static_cast<std::size_t>(d.x) + pbfint::kSweepTileW - 1) /
+                                              pbfint::kSweepTileW);
+        g.first = static_cast<std::uint32_t>(first[k]);
+        g.end = static_cast<std::uint32_t>(first[k + 1]);
+        g.pad = 0;
+        cells += sceneCells;
+        tiles += sceneTiles;
+    }
+    *cellsOut = cells;
+    *tilesOut = tiles;
+    return geom;
+}
+
 }  // namespace
+
+// The grid of the skin lists: the scene's (or every scene's) bounds on
+// cells of h * (1 + skin), the list radius, so the lists keep the 3x3x3
+// stencil.  Built whole before it replaces the old one.
+void PBFSolver::State::make_skin_grid(std::size_t capacity) {
+    GridCore next;
+    DeviceBuffer<std::uint32_t> nextTiles;
+    DeviceBuffer<pbfint::SceneGeom> nextGeom;
+    if (skin > 0.0f && capacity > 0) {
+        const float cs = params.smoothingRadius * (1.0f + skin);
+        if (sceneCount == 0) {
+            next = GridCore::create(capacity, grid.minBounds, grid.maxBounds, cs);
+            nextTiles.allocate(pbfint::sweep_tile_count(next.dims));
+        } else {
+            std::size_t cells = 0, tiles = 0;
+            const std::vector<pbfint::SceneGeom> geom =
+                scene_grids(sceneBoxes, sceneFirst, cs, &cells, &tiles);
+            next = GridCore::createKeyed(capacity, cells, cs);
+            nextTiles.allocate(tiles);
+            nextGeom.allocate(geom.size());
+            nextGeom.copyFromHostToDevice(geom.data(), geom.size());
+        }
+    }
+    drop_graphs();
+    sgrid = std::move(next);
+    stileEnd = std::move(nextTiles);
+    sceneGeomSkin = std::move(nextGeom);
+}
 
 // Allocates the working set, lists and tile ends for maxParticles particles
 // (after the grid is in place); the solver stays inactive until the end.
 void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles) {
-    for (DeviceBuffer<float4>* b : {&x, &xa, &p, &pa, &v, &va, &vin, &vina, &omega})
+    for (DeviceBuffer<float4>* b : {&x, &xa, &p, &pa, &pq, &v, &va, &vin, &vina, &omega, &ckx, &ckv})
         b->allocate(maxParticles_);
     id.allocate(maxParticles_);
     ida.allocate(maxParticles_);
-    status.allocate(1);
+    ckid.allocate(maxParticles_);
+    status.allocate(2);
     nbr.allocate(stride_ * static_cast<std::size_t>(kMaxNeighbors));
     cnt.allocate(maxParticles_);
     stride = stride_;
@@ -729,9 +990,11 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
     s.collision.setContainer(Container{minBounds, maxBounds});
     s.ensure_stream();
     s.allocate(maxParticles, stride, pbfint::sweep_tile_count(s.grid.dims));
+    s.params = params;
+    s.staleFrames = 0;
+    s.make_skin_grid(maxParticles);
     s.maxParticles = maxParticles;
     s.count = 0;
-    s.params = params;
 }
 
 void PBFSolver::setParticles(const float4* positions, const float4* velocities, std::size_t n) {
@@ -798,9 +1061,7 @@ void PBFSolver::step() {
     State& s = *state_;
     if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before stepping");
     if (s.count == 0) return;
-    s.reset_status();
-    s.frame();
-    s.finish();
+    s.run_frames(1);
 }
 
 void PBFSolver::run() {
@@ -813,9 +1074,7 @@ void PBFSolver::run(std::size_t frameCount) {
     State& s = *state_;
     if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before running");
     if (s.count == 0 || frameCount == 0) return;
-    s.reset_status();
-    for (std::size_t f = 0; f < frameCount; ++f) s.frame();
-    s.finish();
+    s.run_frames(frameCount);
 }
 
 void PBFSolver::copyPositionsToHost(float4* out, std::size_t n) const {
@@ -849,6 +1108,106 @@ void PBFSolver::copyVelocitiesToDevice(float4* out, std::size_t n) const {
 std::size_t PBFSolver::particleCount() const { return state_->count; }
 std::size_t PBFSolver::maxParticles() const { return state_->maxParticles; }
 void* PBFSolver::stream() const { return state_->stream; }
+
+void PBFSolver::setNeighborSkin(float skin) {
+    if (!std::isfinite(skin) || skin < 0.0f || skin > 1.0f)
+        throw std::invalid_argument("Neighbour skin must lie in [0, 1]");
+    State& s = *state_;
+    const float old = s.skin;
+    s.skin = skin;
+    try {
+        s.make_skin_grid(s.maxParticles);
+    } catch (...) {
+        s.skin = old;
+        throw;
+    }
+}
+
+float PBFSolver::neighborSkin() const { return state_->skin; }
+std::size_t PBFSolver::staleFrames() const { return state_->staleFrames; }
+
+std::vector<PBFSolver::Tuning> PBFSolver::defaultTunings() {
+    std::vector<Tuning> out;
+    for (float skin : {0.0f, 0.1f, 0.2f, 0.3f})
+        for (int block : {64, 128, 256}) out.push_back(Tuning{skin, block});
+    return out;
+}
+
+namespace {
+void check_tuning(const PBFSolver::Tuning& t) {
+    if (!std::isfinite(t.neighborSkin) || t.neighborSkin < 0.0f || t.neighborSkin > 1.0f)
+        throw std::invalid_argument("Neighbour skin must lie in [0, 1]");
+    if (t.passBlock < 64 || t.passBlock > 256 || t.passBlock % 64 != 0)
+        throw std::invalid_argument("passBlock must be 64, 128, 192 or 256");
+}
+}  // namespace
+
+void PBFSolver::setTuning(const Tuning& tuning) {
+    check_tuning(tuning);
+    setNeighborSkin(tuning.neighborSkin);
+    state_->passBlock = tuning.passBlock;
+    state_->drop_graphs();
+}
+
+PBFSolver::Tuning PBFSolver::tuning() const { return Tuning{state_->skin, state_->passBlock}; }
+
+std::vector<PBFSolver::TuningTime> PBFSolver::autotune(std::size_t frames,
+                                                       const std::vector<Tuning>& candidates) {
+    State& s = *state_;
+    if (s.maxParticles == 0)
+        throw std::logic_error("PBFSolver must be initialized before autotuning");
+    if (frames == 0) throw std::invalid_argument("The autotuner needs at least one frame");
+    const std::vector<Tuning> cands = candidates.empty() ? defaultTunings() : candidates;
+    for (const Tuning& t : cands) check_tuning(t);
+    std::vector<TuningTime> times;
+    if (s.count == 0) return times;
+    const Tuning original = tuning();
+    const std::size_t stale0 = s.staleFrames;
+    const std::size_t n = s.count;
+    DeviceBuffer<float4> sx, sv;
+    DeviceBuffer<std::uint32_t> sid;
+    sx.allocate(n);
+    sv.allocate(n);
+    sid.allocate(n);
+    const auto copy = [&s, n](void* dst, const void* src, std::size_t bytes) {
+        hip_check(hipMemcpyAsync(dst, src, n * bytes, hipMemcpyDeviceToDevice, s.stream));
+    };
+    copy(sx.data(), s.x.data(), sizeof(float4));
+    copy(sv.data(), s.v.data(), sizeof(float4));
+    copy(sid.data(), s.id.data(), sizeof(std::uint32_t));
+    const auto restore = [&] {
+        copy(s.x.data(), sx.data(), sizeof(float4));
+        copy(s.v.data(), sv.data(), sizeof(float4));
+        copy(s.id.data(), sid.data(), sizeof(std::uint32_t));
+        hip_check(hipStreamSynchronize(s.stream));
+    };
+    try {
+        for (const Tuning& t : cands) {
+            setTuning(t);
+            restore();
+            s.run_frames(1);  // first use: graph capture, caches
+            restore();
+            const auto t0 = std::chrono::steady_clock::now();
+            s.run_frames(frames);
+            const auto t1 = std::chrono::steady_clock::now();
+            times.push_back(TuningTime{
+                t, std::chrono::duration<double, std::milli>(t1 - t0).count() /
+                       static_cast<double>(frames)});
+        }
+    } catch (...) {
+        restore();
+        setTuning(original);
+        s.staleFrames = stale0;
+        throw;
+    }
+    const TuningTime* best = &times.front();
+    for (const TuningTime& tt : times)
+        if (tt.msPerFrame < best->msPerFrame) best = &tt;
+    restore();
+    setTuning(best->tuning);
+    s.staleFrames = stale0;
+    return times;
+}
 
 void PBFSolver::setGraphs(bool enabled) {
     state_->useGraphs = enabled;
@@ -920,10 +1279,8 @@ void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
     // Every scene's grid exactly as PBFSolver::initialize builds it (the
     // bounds checks of spatial_grid.cu:11-27 included), laid end to end.
     const float h = params.smoothingRadius;
-    std::vector<pbfint::SceneGeom> geom(scenes.size());
     std::vector<pbf_container> boxes(scenes.size());
     std::vector<std::size_t> first(scenes.size() + 1, 0);
-    std::size_t cells = 0, tiles = 0;
     for (std::size_t k = 0; k < scenes.size(); ++k) {
         const BatchScene& sc = scenes[k];
         const float3 lo = sc.minBounds, hi = sc.maxBounds;
@@ -932,27 +1289,11 @@ void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
             !std::isfinite(hi.x) || !std::isfinite(hi.y) || !std::isfinite(hi.z) ||
             hi.x <= lo.x || hi.y <= lo.y || hi.z <= lo.z)
             throw std::invalid_argument("Invalid spatial grid bounds");
-        std::size_t sceneCells = 0;
-        const int3 d = GridCore::dimsFor(lo, hi, h, &sceneCells);
-        const std::size_t sceneTiles = pbfint::sweep_tile_count(d);
-        if (cells + sceneCells > std::numeric_limits<std::uint32_t>::max() ||
-            tiles + sceneTiles > std::numeric_limits<std::uint32_t>::max())
-            throw std::length_error("Batched grids exceed 32-bit key capacity");
-        pbfint::SceneGeom& g = geom[k];
-        g.minBounds = lo;
-        g.dims = d;
-        g.cellOff = static_cast<std::uint32_t>(cells);
-        g.tileOff = static_cast<std::uint32_t>(tiles);
-        g.chunks = static_cast<std::uint32_t>((static_cast<std::size_t>(d.x) + pbfint::kSweepTileW - 1) /
-                                              pbfint::kSweepTileW);
-        g.first = static_cast<std::uint32_t>(first[k]);
-        g.end = static_cast<std::uint32_t>(first[k] + sc.particleCount);
-        g.pad = 0;
         boxes[k] = pbf_container{lo, hi};
         first[k + 1] = first[k] + sc.particleCount;
-        cells += sceneCells;
-        tiles += sceneTiles;
     }
+    std::size_t cells = 0, tiles = 0;
+    const std::vector<pbfint::SceneGeom> geom = scene_grids(boxes, first, h, &cells, &tiles);
     std::vector<std::uint32_t> sceneOf(total);
     for (std::size_t k = 0; k < scenes.size(); ++k)
         for (std::size_t i = first[k]; i < first[k + 1]; ++i)
@@ -975,9 +1316,11 @@ void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
     s.sceneOf.allocate(total);
     s.sceneOf.copyFromHostToDevice(sceneOf.data(), total);
     s.sceneFirst = std::move(first);
+    s.sceneBoxes = std::move(boxes);
     s.sceneCount = scenes.size();
-    s.maxParticles = total;
     s.params = params;
+    s.make_skin_grid(total);
+    s.maxParticles = total;
 }
 
 void PBFBatchSolver::setParticles(const float4* positions, const float4* velocities,
@@ -1020,6 +1363,16 @@ std::size_t PBFBatchSolver::sceneFirst(std::size_t scene) const {
 }
 
 void PBFBatchSolver::setGraphs(bool enabled) { solver_.setGraphs(enabled); }
+void PBFBatchSolver::setNeighborSkin(float skin) { solver_.setNeighborSkin(skin); }
+std::size_t PBFBatchSolver::staleFrames() const { return solver_.staleFrames(); }
+void PBFBatchSolver::setTuning(const PBFSolver::Tuning& tuning) { solver_.setTuning(tuning); }
+PBFSolver::Tuning PBFBatchSolver::tuning() const { return solver_.tuning(); }
+std::vector<PBFSolver::TuningTime> PBFBatchSolver::autotune(
+    std::size_t frames, const std::vector<PBFSolver::Tuning>& candidates) {
+    if (solver_.state_->sceneCount == 0)
+        throw std::logic_error("PBFBatchSolver must be initialized before autotuning");
+    return solver_.autotune(frames, candidates);
+}
 void PBFBatchSolver::setProfiling(bool enabled) { solver_.setProfiling(enabled); }
 std::vector<PBFSolver::KernelTime> PBFBatchSolver::kernelTimes() const {
     return solver_.kernelTimes();
@@ -1080,6 +1433,21 @@ struct PBFAdjoint::Impl {
 PBFAdjoint::PBFAdjoint(PBFSolver& solver) : impl_(new Impl()) { impl_->solver = &solver; }
 PBFAdjoint::~PBFAdjoint() = default;
 
+namespace {
+// Runs f when the scope ends.
+template <class F>
+class OnExit {
+public:
+    explicit OnExit(F f) : f_(f) {}
+    ~OnExit() { f_(); }
+    OnExit(const OnExit&) = delete;
+    OnExit& operator=(const OnExit&) = delete;
+
+private:
+    F f_;
+};
+}  // namespace
+
 void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
     Impl& m = *impl_;
     PBFSolver::State& s = m.S();
@@ -1095,6 +1463,11 @@ void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
     m.every = checkpointEvery;
     m.n = s.count;
     m.checkpoints.clear();
+    // The adjoint's tapes and reverse kernels follow the reference's five
+    // grid builds per substep: no skin lists while it runs frames.
+    const bool wasExact = s.adjointExact;
+    s.adjointExact = true;
+    const OnExit restore([&s, wasExact] { s.adjointExact = wasExact; });
     m.tapedSegment = static_cast<std::size_t>(-1);
     s.reset_status();
     // The last segment is taped as it runs: backward() starts with it, and
@@ -1138,6 +1511,9 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
         throw std::out_of_range("Requested adjoint count does not match active particles");
     const std::size_t n = m.n;
     const SimulationParams& P = s.params;
+    const bool wasExact = s.adjointExact;
+    s.adjointExact = true;
+    const OnExit restore([&s, wasExact] { s.adjointExact = wasExact; });
     for (DeviceBuffer<float4>* b :
          {&m.pb, &m.vb, &m.x0b, &m.v1b, &m.v3b, &m.etab, &m.wb, &m.prb, &m.gsum})
         b->allocate(n);

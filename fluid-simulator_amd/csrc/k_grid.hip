@@ -405,14 +405,19 @@ int radix_key_bits(size_t numCells) {
     return bits;
 }
 
-// Stable sort of (key, slot) pairs by the low `bits` key bits.
-// kin/tmpK may alias (kin is consumed by the first pass); vin == nullptr
-// means the values are the slot indices 0..n-1.  The result lands in
-// kout/vout.  scratch holds radix_scratch_words(n) words.
+// Stable sort of (key, slot) pairs by the low `bits` key bits.  vin ==
+// nullptr means the values are the slot indices 0..n-1.  The passes run
+// from the inputs through tmpK/tmpV and kout/vout in turn, ending in
+// kout/vout; a pass never writes the array it reads (its tiles are read
+// and written by different blocks), so the five arrays must be distinct.
+// scratch holds radix_scratch_words(n) words.
 pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                             uint32_t* vout, uint32_t* tmpK, uint32_t* tmpV, size_t n, int bits,
                             uint32_t* scratch, void* stream) {
     if (n == 0) return PBF_OK;
+    if (kin == tmpK || kin == kout || (vin != nullptr && (vin == tmpV || vin == vout)) ||
+        kout == tmpK || vout == tmpV)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "radix sort buffers must not alias");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (bits <= 0) {
         hipLaunchKernelGGL(k_sort_identity, dim3(grid_for(n)), dim3(kBlock), 0, s, kin, kout, vout,

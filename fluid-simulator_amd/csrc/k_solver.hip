@@ -85,12 +85,15 @@ __global__ void __launch_bounds__(kBlock) k_integrate(const pbf_float4* __restri
 // neighbour's position and lambda in one 16-byte gather.  Storing .w while
 // other lanes read .xyz of the same records touches disjoint bytes, hence
 // neither pointer is restrict-qualified.
-template <bool kFast>
+// kSkin: the list is a skin list; its entries are tested against h (the
+// reference's inclusive FP32 test), and more than exactMax within h raise
+// the overflow bit, as the grid build the skin list stands in for would.
+template <bool kFast, bool kSkin = false>
 __global__ void __launch_bounds__(kBlock)
     k_lambda_list(const pbf_float4* p, const uint32_t* __restrict__ nbr,
                   const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
                   float eps, float* lambda, size_t lstride, size_t stride,
-                  const uint32_t* __restrict__ status) {
+                  uint32_t* __restrict__ status, int exactMax) {
     const size_t i = thread_index<kFast>();
     if (i >= n) return;
     if (status != nullptr && *status != 0u) return;
@@ -107,11 +110,18 @@ __global__ void __launch_bounds__(kBlock)
     // FP32 sum's significant bits.
     double rho = static_cast<double>(mass * s.poly_norm);
     double gix = 0.0, giy = 0.0, giz = 0.0, sg = 0.0;
+    int within = 0;
     for_each_neighbour<kFast>(
         nbr, stride, i, c, [&](uint32_t j) { return ld4(p, j); },
         [&](const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
+            float d2;
+            if (kSkin) {
+                d2 = dist2_exact(dx, dy, dz);
+                within += d2 <= s.h2 ? 1 : 0;
+            } else {
+                d2 = dx * dx + dy * dy + dz * dz;
+            }
             rho += static_cast<double>(mass * poly6_d2(d2, s));
             const float3 gw = grad<kFast>(dx, dy, dz, d2, s);
             const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
@@ -123,6 +133,7 @@ __global__ void __launch_bounds__(kBlock)
     sg += gix * gix + giy * giy + giz * giz;
     const double C = rho / static_cast<double>(rho0) - 1.0;
     lambda[i * lstride] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
+    if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
 }
 
 // computeDeltaPosition (src/pbf/pbf_solver_kernels.cu:108-182).
@@ -231,7 +242,8 @@ __global__ void __launch_bounds__(kBlock)
     k_vorticity_list(const pbf_float4* __restrict__ p, const pbf_float4* __restrict__ v,
                      const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                      size_t n, float h, pbf_float4* __restrict__ omega,
-                     pbf_float4* __restrict__ pw, size_t stride) {
+                     pbf_float4* __restrict__ pw, size_t stride, uint32_t* __restrict__ status,
+                     int exactMax) {
     const size_t i = thread_index<kFast>();
     if (i >= n) return;
     const int c = cnt[i];
@@ -244,17 +256,20 @@ __global__ void __launch_bounds__(kBlock)
     const Sph s = sph_consts(h);
     const float4 vi = ld4(v, i);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    int within = 0;
     for_each_neighbour<kFast>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(v, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &vj = r.b;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            if (exactMax > 0) within += dist2_exact(dx, dy, dz) <= s.h2 ? 1 : 0;
             const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
             ox += g.y * uz - g.z * uy;
             oy += g.z * ux - g.x * uz;
             oz += g.x * uy - g.y * ux;
         });
+    if (exactMax > 0 && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
     const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
     if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, wl));
@@ -457,13 +472,14 @@ __global__ void k_scorr(const float* __restrict__ d2, size_t n, float h, float k
 
 // integrate + collisionInputVelocities snapshot + cell key of p*
 // (pbf_solver.cu:199-208 and spatial_grid.cu:100-120 in one pass).
+// With box set (skin lists), the key is that of p*'s container point.
 template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_predict_key(const pbf_float4* __restrict__ x, pbf_float4* __restrict__ p,
                   pbf_float4* __restrict__ v, pbf_float4* __restrict__ vin,
                   uint32_t* __restrict__ key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                  float cs, pbf_int3 gs, pbfint::SceneTable st,
-                  const uint32_t* __restrict__ status) {
+                  float cs, pbf_int3 gs, const pbf_container* __restrict__ box, float radius,
+                  pbfint::SceneTable st, const uint32_t* __restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (*status != 0u) return;
@@ -480,7 +496,31 @@ __global__ void __launch_bounds__(kBlock)
     st4(v, i, vi);
     st4(vin, i, vi);
     st4(p, i, pi);
-    key[i] = cell_key<kBatched>(pi, static_cast<size_t>(i), mn, cs, gs, st);
+    float4 kp = pi;
+    if (box != nullptr) {
+        const pbf_container& c = kBatched ? st.containers[st.sceneOf[i]] : *box;
+        const float3 q = container_point(pi, c, radius);
+        kp = make_float4(q.x, q.y, q.z, pi.w);
+    }
+    key[i] = cell_key<kBatched>(kp, static_cast<size_t>(i), mn, cs, gs, st);
+}
+
+// The frame checkpoint of the skin lists: the state a stale frame restarts from.
+__global__ void __launch_bounds__(kBlock)
+    k_checkpoint(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ v,
+                 const uint32_t* __restrict__ id, pbf_float4* __restrict__ cx,
+                 pbf_float4* __restrict__ cv, uint32_t* __restrict__ cid, size_t n,
+                 const uint32_t* __restrict__ status) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (*status != 0u) return;
+    st4(cx, i, ld4(x, i));
+    st4(cv, i, ld4(v, i));
+    cid[i] = id[i];
+}
+
+__global__ void k_frame_done(uint32_t* __restrict__ status) {
+    if (threadIdx.x == 0 && status[0] == 0u) status[1] += 1u;
 }
 
 // Cell keys of the current working set (computeParticleKeys,
@@ -500,14 +540,18 @@ __global__ void __launch_bounds__(kBlock)
 // positions the reference's separate Delta-p kernel sees.  pin[].w holds
 // lambda (stored there by the lambda pass); pout[].w gets back the
 // particle's own .w, which x carries.
-template <bool kBatched>
+// kSkin: the list is a skin list (out-of-range entries add exact zeros),
+// and a particle that ends farther than sqrt(stale2) from the container
+// point of its p0 (the position the lists were built around) marks the
+// lists stale.
+template <bool kBatched, bool kSkin>
 __global__ void __launch_bounds__(kBlock)
     k_delta_apply_collide(const pbf_float4* __restrict__ pin, pbf_float4* __restrict__ pout,
                           const uint32_t* __restrict__ nbr, const int* __restrict__ cnt,
                           int maxN, const pbf_float4* __restrict__ x, size_t n, float h,
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
                           Colliders col, float radius, pbfint::SceneTable st,
-                          uint32_t* __restrict__ status) {
+                          const pbf_float4* p0, float stale2, uint32_t* __restrict__ status) {
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -541,9 +585,15 @@ __global__ void __launch_bounds__(kBlock)
     pi.y += dpy;
     pi.z += dpz;
     pi.w = x[i].w;
-    if (!collision_fixed_point(pi, colliders_of<kBatched>(col, i, st), radius))
+    const Colliders ci = colliders_of<kBatched>(col, i, st);
+    if (!collision_fixed_point(pi, ci, radius))
         atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
     st4(pout, i, pi);
+    if (kSkin) {
+        const float3 q = container_point(ld4(p0, i), *ci.container, radius);
+        const float ex = pi.x - q.x, ey = pi.y - q.y, ez = pi.z - q.z;
+        if (!(ex * ex + ey * ey + ez * ez <= stale2)) atomicOr(status, PBF_STATUS_LIST_STALE);
+    }
 }
 
 // updateVelocityAndPosition + applyXsphViscosity (pbf_solver.cu:284-298).
@@ -551,12 +601,15 @@ __global__ void __launch_bounds__(kBlock)
 // neighbour with the same FP32 expression the reference stores, so the
 // stored intermediate velocity buffer is not needed.  x <- p* is a buffer
 // swap on the host.
+// kSkin: skin lists; more than exactMax neighbours within h raise the
+// overflow bit (the post-solve build's check).
+template <bool kSkin>
 __global__ void __launch_bounds__(kBlock)
     k_update_xsph(const pbf_float4* __restrict__ x, const pbf_float4* __restrict__ p,
                   const pbf_float4* __restrict__ vold, pbf_float4* __restrict__ vout,
                   const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                   size_t n, float h, float visc, float inv_dt, size_t stride,
-                  const uint32_t* __restrict__ status) {
+                  uint32_t* __restrict__ status, int exactMax) {
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -573,6 +626,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     const Sph s = sph_consts(h);
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    int within = 0;
     for_each_neighbour<true>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
         [&](const PosPair& r) {
@@ -581,12 +635,20 @@ __global__ void __launch_bounds__(kBlock)
             const float vjy = (pj.y - xj.y) * inv_dt;
             const float vjz = (pj.z - xj.z) * inv_dt;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
+            float d2;
+            if (kSkin) {
+                d2 = dist2_exact(dx, dy, dz);
+                within += d2 <= s.h2 ? 1 : 0;
+            } else {
+                d2 = dx * dx + dy * dy + dz * dz;
+            }
+            const float w = poly6_d2(d2, s);
             cx += (vjx - vix) * w;
             cy += (vjy - viy) * w;
             cz += (vjz - viz) * w;
         });
     st4(vout, i, make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw));
+    if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
 }
 
 // updateVelocityAndPosition without XSPH, velocity only (x <- p* is a swap).
@@ -685,7 +747,7 @@ pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const 
                                 float* lambda, size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_lambda_list<false>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, size_t{1}, stride, static_cast<const uint32_t*>(nullptr));
+               rho0, eps, lambda, size_t{1}, stride, static_cast<uint32_t*>(nullptr), 0);
 }
 
 pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
@@ -721,7 +783,8 @@ pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, con
                                    size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_vorticity_list<false>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, static_cast<pbf_float4*>(nullptr), stride);
+               omega, static_cast<pbf_float4*>(nullptr), stride, static_cast<uint32_t*>(nullptr),
+               0);
 }
 
 pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
@@ -817,13 +880,24 @@ namespace pbfint {
 
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                              float cs, pbf_int3 gs, const SceneTable& st,
-                              const uint32_t* status, void* stream) {
+                              float cs, pbf_int3 gs, const pbf_container* box, float radius,
+                              const SceneTable& st, const uint32_t* status, void* stream) {
     if (st.geom != nullptr)
         PBF_LAUNCH(k_predict_key<true>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g,
-                   mn, cs, gs, st, status);
+                   mn, cs, gs, box, radius, st, status);
     PBF_LAUNCH(k_predict_key<false>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g, mn,
-               cs, gs, st, status);
+               cs, gs, box, radius, st, status);
+}
+
+pbf_status launch_checkpoint(const pbf_float4* x, const pbf_float4* v, const uint32_t* id,
+                             pbf_float4* cx, pbf_float4* cv, uint32_t* cid, size_t n,
+                             const uint32_t* status, void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_checkpoint, grid_for(n), kBlock, stream, x, v, id, cx, cv, cid, n, status);
+}
+
+pbf_status launch_frame_done(uint32_t* status, void* stream) {
+    PBF_LAUNCH(k_frame_done, 1, 64, stream, status);
 }
 
 pbf_status launch_collide_fixed_point(pbf_float4* p, size_t n, const Colliders& col, float radius,
@@ -845,30 +919,43 @@ pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 
 
 pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
                                  size_t n, float h, float mass, float rho0, float eps,
-                                 size_t stride, const uint32_t* status, void* stream) {
+                                 size_t stride, const ListPass& lp, uint32_t* status, void* stream) {
     float* lambda = reinterpret_cast<float*>(p) + 3;  // the .w words
-    PBF_LAUNCH(k_lambda_list<true>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, size_t{4}, stride, status);
+    if (lp.p0 != nullptr)
+        PBF_LAUNCH((k_lambda_list<true, true>), grid_for(n, lp.block), lp.block, stream, p, nbr, cnt, maxN, n,
+                   h, mass, rho0, eps, lambda, size_t{4}, stride, status, lp.exactMax);
+    PBF_LAUNCH((k_lambda_list<true, false>), grid_for(n, lp.block), lp.block, stream, p, nbr, cnt, maxN, n, h,
+               mass, rho0, eps, lambda, size_t{4}, stride, status, 0);
 }
 
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
                                       const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const Colliders& col, float radius,
-                                      const SceneTable& st, uint32_t* status, void* stream) {
-    if (st.geom != nullptr)
-        PBF_LAUNCH(k_delta_apply_collide<true>, grid_for(n), kBlock, stream, pin, pout, nbr, cnt,
-                   maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, status);
-    PBF_LAUNCH(k_delta_apply_collide<false>, grid_for(n), kBlock, stream, pin, pout, nbr, cnt,
-               maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, status);
+                                      const SceneTable& st, const ListPass& lp, uint32_t* status,
+                                      void* stream) {
+    const bool batched = st.geom != nullptr;
+    const bool skinned = lp.p0 != nullptr;
+#define PBF_DELTA(B, S)                                                                        \
+    PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin, pout, nbr, cnt, \
+               maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, lp.p0,      \
+               lp.stale2, status)
+    if (batched && skinned) PBF_DELTA(true, true);
+    if (batched) PBF_DELTA(true, false);
+    if (skinned) PBF_DELTA(false, true);
+    PBF_DELTA(false, false);
+#undef PBF_DELTA
 }
 
 pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
                               pbf_float4* vout, const uint32_t* nbr, const int* cnt, int maxN,
                               size_t n, float h, float visc, float inv_dt, size_t stride,
-                              const uint32_t* status, void* stream) {
-    PBF_LAUNCH(k_update_xsph, grid_for(n), kBlock, stream, x, p, vold, vout, nbr, cnt, maxN, n, h,
-               visc, inv_dt, stride, status);
+                              const ListPass& lp, uint32_t* status, void* stream) {
+    if (lp.exactMax > 0)
+        PBF_LAUNCH(k_update_xsph<true>, grid_for(n, lp.block), lp.block, stream, x, p, vold, vout, nbr, cnt,
+                   maxN, n, h, visc, inv_dt, stride, status, lp.exactMax);
+    PBF_LAUNCH(k_update_xsph<false>, grid_for(n, lp.block), lp.block, stream, x, p, vold, vout, nbr, cnt, maxN,
+               n, h, visc, inv_dt, stride, status, 0);
 }
 
 pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
@@ -879,9 +966,10 @@ pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p,
 
 pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
-                            pbf_float4* pw, size_t stride, void* stream) {
-    PBF_LAUNCH(k_vorticity_list<true>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, pw, stride);
+                            pbf_float4* pw, size_t stride, const ListPass& lp, uint32_t* status,
+                            void* stream) {
+    PBF_LAUNCH(k_vorticity_list<true>, grid_for(n, lp.block), lp.block, stream, p, v, nbr, cnt, maxN, n, h,
+               omega, pw, stride, status, lp.exactMax);
 }
 
 pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,
@@ -889,12 +977,13 @@ pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, con
                                   const pbf_float4* vcur, const pbf_float4* vsnap, pbf_float4* vout,
                                   size_t n, float h, float dt, float strength, size_t stride,
                                   const Colliders& col, float radius, float e, float f,
-                                  const SceneTable& st, const uint32_t* status, void* stream) {
+                                  const SceneTable& st, const ListPass& lp,
+                                  const uint32_t* status, void* stream) {
     if (st.geom != nullptr)
-        PBF_LAUNCH(k_confine_resolve<true>, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN,
+        PBF_LAUNCH(k_confine_resolve<true>, grid_for(n, lp.block), lp.block, stream, x, pw, nbr, cnt, maxN,
                    omega, vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st,
                    status);
-    PBF_LAUNCH(k_confine_resolve<false>, grid_for(n), kBlock, stream, x, pw, nbr, cnt, maxN, omega,
+    PBF_LAUNCH(k_confine_resolve<false>, grid_for(n, lp.block), lp.block, stream, x, pw, nbr, cnt, maxN, omega,
                vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st, status);
 }
 

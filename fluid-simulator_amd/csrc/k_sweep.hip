@@ -31,8 +31,13 @@
 //     of its own that reads its 27 cells from global memory instead of
 //     sweeping the tiles one after another.
 //
-// Requires a 3x3x3 stencil (cell size == smoothing radius), which the
-// solver's grid guarantees (src/pbf/pbf_solver.cu:81-82).
+// Requires a 3x3x3 stencil (cell size >= the list radius), which the
+// solver's grids guarantee (src/pbf/pbf_solver.cu:81-82 for the reference's
+// cells of size h).
+//
+// Skin lists (the solver's one build per substep) use the same sweep with
+// radius h * (1 + skin) over cells of that size, every candidate taken at
+// the container point of its position.
 #include "pbf_device.hpp"
 #include "pbf_internal.hpp"
 #include "pbf_launch.hpp"
@@ -65,6 +70,12 @@ struct Grid {
     float h2;
     int maxNeighbors;
     pbfint::SceneTable st;  // a batched step's scenes
+    // Skin lists: candidates are taken at their container points, and a
+    // list past maxNeighbors marks the lists stale instead of overflowing.
+    bool skin;
+    pbf_container box;
+    float radius;
+    uint32_t fullBit;
 };
 
 // The grid a tile or particle belongs to: the one grid of a single-scene
@@ -75,13 +86,24 @@ struct Geom {
     uint32_t cellOff;
     uint32_t tileOff;
     int chunks;
+    pbf_container box;  // the scene's container (skin lists)
 };
 
 template <bool kBatched>
 __device__ __forceinline__ Geom geom_of(const Grid& g, size_t slot) {
-    if (!kBatched) return Geom{g.dims, 0u, 0u, g.chunks};
-    const pbfint::SceneGeom& sg = g.st.geom[g.st.sceneOf[slot]];
-    return Geom{sg.dims, sg.cellOff, sg.tileOff, static_cast<int>(sg.chunks)};
+    if (!kBatched) return Geom{g.dims, 0u, 0u, g.chunks, g.box};
+    const uint32_t scene = g.st.sceneOf[slot];
+    const pbfint::SceneGeom& sg = g.st.geom[scene];
+    return Geom{sg.dims, sg.cellOff, sg.tileOff, static_cast<int>(sg.chunks),
+                g.skin ? g.st.containers[scene] : pbf_container{}};
+}
+
+// The position a candidate is tested at: its own, or for skin lists its
+// container point (the keys of a skin build are those of the same points).
+template <bool kSkin>
+__device__ __forceinline__ float3 test_point(const pbf_container& box, float radius, float4 p) {
+    if (kSkin) return container_point(p, box, radius);
+    return make_float3(p.x, p.y, p.z);
 }
 
 struct Meta {
@@ -141,13 +163,15 @@ struct LdsRow {
     }
 };
 
+template <bool kSkin>
 struct GlobalRow {
     const pbf_float4* p;
     size_t first;
     int last;
+    pbf_container box;
+    float radius;
     __device__ float3 position(int q) const {
-        const float4 v = ld4(p, first + static_cast<size_t>(min(q, last)));
-        return make_float3(v.x, v.y, v.z);
+        return test_point<kSkin>(box, radius, ld4(p, first + static_cast<size_t>(min(q, last))));
     }
 };
 
@@ -246,12 +270,13 @@ __device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const 
         g.nbr[static_cast<size_t>(st.stored >> 2) * g.stride + gi] = last;
     }
     g.cnt[gi] = st.count;
-    if (st.count > g.maxNeighbors) atomicOr(g.status, PBF_STATUS_NEIGHBOR_OVERFLOW);
+    if (st.count > g.maxNeighbors) atomicOr(g.status, g.fullBit);
 }
 
 // One tile: segment geometry, then the tile's particles against the three
 // z-planes of its halo.  Every branch that reaches a barrier is uniform
 // over the workgroup (it depends on LDS values written before a barrier).
+template <bool kSkin>
 __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& m, Plane& lds,
                                            int tile) {
     const int local = tile - static_cast<int>(gm.tileOff);
@@ -323,8 +348,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& 
         int mc = 0;
         ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
         if (active) {
-            const float4 p = ld4(g.p, gi);
-            ip = make_float3(p.x, p.y, p.z);
+            ip = test_point<kSkin>(gm.box, g.radius, ld4(g.p, gi));
             mc = static_cast<int>((g.sortedKeys[gi] - gm.cellOff) %
                                   static_cast<uint32_t>(gm.dims.x)) - xs;
         }
@@ -341,7 +365,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& 
                     const size_t first = static_cast<size_t>(m.segFirst[s]);
                     const int base = m.segBase[s];
                     for (int q = tid; q < cnt; q += kThreads) {
-                        const float4 a = ld4(g.p, first + q);
+                        const float3 a = test_point<kSkin>(gm.box, g.radius, ld4(g.p, first + q));
                         float* r = lds.xyz + 3 * (base + q);
                         r[0] = a.x;
                         r[1] = a.y;
@@ -364,7 +388,8 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& 
                     const LdsRow src{&lds, m.segBase[s]};
                     sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
                 } else {
-                    const GlobalRow src{g.p, first, m.cellOff[s][nc] - 1};
+                    const GlobalRow<kSkin> src{g.p, first, m.cellOff[s][nc] - 1, gm.box,
+                                               g.radius};
                     sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
                 }
             }
@@ -377,7 +402,7 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& 
 // One particle on one lane, candidates read from global memory: the row
 // range of every (dz, dy) is the run of particles from the first non-empty
 // cell of cx-1 .. cx+1 to the last, as in the tile form.
-template <bool kBatched>
+template <bool kBatched, bool kSkin>
 __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
     const Geom gm = geom_of<kBatched>(g, gi);
     const uint32_t key = g.sortedKeys[gi] - gm.cellOff;
@@ -386,8 +411,7 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
     const uint32_t rowId = key / gx;
     const int cy = static_cast<int>(rowId % static_cast<uint32_t>(gm.dims.y));
     const int cz = static_cast<int>(rowId / static_cast<uint32_t>(gm.dims.y));
-    const float4 p = ld4(g.p, gi);
-    const float3 ip = make_float3(p.x, p.y, p.z);
+    const float3 ip = test_point<kSkin>(gm.box, g.radius, ld4(g.p, gi));
     ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
     const int xa = max(cx - 1, 0);
     const int xb = min(cx + 1, gm.dims.x - 1);
@@ -406,7 +430,8 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
         if (rowStart < 0) continue;
         const int len = rowEnd - rowStart;
         const int selfq = s == 4 ? static_cast<int>(gi - static_cast<size_t>(rowStart)) : -1;
-        const GlobalRow src{g.p, static_cast<size_t>(rowStart), len - 1};
+        const GlobalRow<kSkin> src{g.p, static_cast<size_t>(rowStart), len - 1, gm.box,
+                                   g.radius};
         sweep_row(g, src, ip, 0, len, selfq, gi, static_cast<size_t>(rowStart), st);
     }
     finish_particle(g, gi, st);
@@ -417,7 +442,7 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
 // waves per SIMD (72 VGPRs, three spilled on cold paths) rather than the six
 // that 77 VGPRs allow: the sweep waits on LDS and the list stores most of
 // the time, and the seventh wave hides part of it.
-template <bool kBatched>
+template <bool kBatched, bool kSkin>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_build_lists(Grid g) {
     __shared__ Meta m;
@@ -458,14 +483,14 @@ k_build_lists(Grid g) {
     lds_barrier();
     if (total <= kMaxTilePasses) {
         for (int k = 0; k < total; ++k)
-            sweep_tile(g, geom_of<kBatched>(g, static_cast<size_t>(m.tileFirst[k])), m, lds,
-                       m.tiles[k]);
+            sweep_tile<kSkin>(g, geom_of<kBatched>(g, static_cast<size_t>(m.tileFirst[k])), m,
+                              lds, m.tiles[k]);
     } else {
         // The particles of every tile that starts here, one lane each.
         const size_t first = static_cast<size_t>(m.tileFirst[0]);
         const size_t end = g.tileEnd[m.tiles[total - 1]];
         for (size_t q = first + static_cast<size_t>(tid); q < end; q += kThreads)
-            sweep_lane<kBatched>(g, q);
+            sweep_lane<kBatched, kSkin>(g, q);
     }
 }
 
@@ -503,15 +528,24 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
     g.h2 = sg.h * sg.h;
     g.maxNeighbors = sg.maxNeighbors;
     g.st = sg.st;
+    g.skin = sg.skin;
+    g.box = sg.box != nullptr ? *sg.box : pbf_container{};
+    g.radius = sg.radius;
+    g.fullBit = sg.skin ? PBF_STATUS_LIST_STALE : PBF_STATUS_NEIGHBOR_OVERFLOW;
+    if (sg.skin && sg.st.geom == nullptr && sg.box == nullptr)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "skin lists need the container");
     const size_t tiles = sg.st.geom != nullptr ? 0 : sweep_tile_count(sg.dims);
     if (tiles > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "grid has too many rows for the neighbour sweep");
     const size_t tasks = (sg.n + kThreads - 1) / kThreads;
     if (tasks > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "too many particles for the neighbour sweep");
-    if (sg.st.geom != nullptr)
-        PBF_LAUNCH(k_build_lists<true>, static_cast<unsigned>(tasks), kThreads, stream, g);
-    PBF_LAUNCH(k_build_lists<false>, static_cast<unsigned>(tasks), kThreads, stream, g);
+    const unsigned nb = static_cast<unsigned>(tasks);
+    if (sg.st.geom != nullptr && sg.skin)
+        PBF_LAUNCH((k_build_lists<true, true>), nb, kThreads, stream, g);
+    if (sg.st.geom != nullptr) PBF_LAUNCH((k_build_lists<true, false>), nb, kThreads, stream, g);
+    if (sg.skin) PBF_LAUNCH((k_build_lists<false, true>), nb, kThreads, stream, g);
+    PBF_LAUNCH((k_build_lists<false, false>), nb, kThreads, stream, g);
 }
 
 }  // namespace pbfint

@@ -310,6 +310,15 @@ __device__ __forceinline__ bool project_plane(float4& p, const pbf_plane& pl, fl
     return true;
 }
 
+// The container projection of p's coordinates (project_container's
+// arithmetic).  Skin lists are built around these points: the projection is
+// a per-axis clamp, so it never moves two particles farther apart, and a
+// pair within h before it is within h after it.
+__device__ __forceinline__ float3 container_point(float4 p, const pbf_container& c, float r) {
+    project_container(p, c, r);
+    return make_float3(p.x, p.y, p.z);
+}
+
 // One pass of CollisionSystem::solve for one particle (collision_system.cu:150-169):
 // container, then spheres, boxes, planes, each kernel in array order.
 __device__ __forceinline__ bool collision_pass(float4& p, const Colliders& c, float r) {

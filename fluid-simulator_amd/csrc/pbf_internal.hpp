@@ -32,6 +32,22 @@ struct SceneTable {
     const uint32_t* sceneOf = nullptr;  // scene of every slot
 };
 
+// Skin lists: one grid build per substep, around the container points of
+// the predicted positions, with radius h * (1 + skin); every pass tests each
+// entry against h itself.  The lists stay complete while no particle has
+// moved more than half the skin from its point (the Delta-p pass checks and
+// raises PBF_STATUS_LIST_STALE); the counts within h replace the counts of
+// the builds they stand in for.
+// A neighbour pass's launch: the skin-list bookkeeping above and the
+// workgroup size (threads, a multiple of 64 up to 256; results do not
+// depend on it).
+struct ListPass {
+    const pbf_float4* p0 = nullptr;  // predicted positions the lists were built around
+    float stale2 = 0.0f;             // squared displacement from their points that is too far
+    int exactMax = 0;                // > 0: neighbours within h above this raise the overflow bit
+    int block = 256;
+};
+
 // Grid (k_grid.hip)
 long long stencil_radius(float h, float cellSize);
 pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* ce, pbf_int3 g,
@@ -49,45 +65,62 @@ pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pb
                               uint32_t* tileEnd, const SceneTable& st, void* stream);
 size_t radix_scratch_words(size_t n);
 int radix_key_bits(size_t numCells);
+// Stable sort of (key, value) pairs by the low `bits` key bits into
+// kout/vout (vin == nullptr: values are the slots); the five arrays are
+// distinct.
 pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                             uint32_t* vout, uint32_t* tmpK, uint32_t* tmpV, size_t n, int bits,
                             uint32_t* scratch, void* stream);
 
 // Solver (k_solver.hip)
+// With box not null, the keys are those of the container points of p*
+// (skin lists); box is the single scene's container (a batched step uses
+// the scene table's).
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
-                              float cs, pbf_int3 gs, const SceneTable& st,
-                              const uint32_t* status, void* stream);
+                              float cs, pbf_int3 gs, const pbf_container* box, float radius,
+                              const SceneTable& st, const uint32_t* status, void* stream);
+// Copies x, v and id to the frame checkpoint while *status is zero.
+pbf_status launch_checkpoint(const pbf_float4* x, const pbf_float4* v, const uint32_t* id,
+                             pbf_float4* cx, pbf_float4* cv, uint32_t* cid, size_t n,
+                             const uint32_t* status, void* stream);
+// Counts a finished frame in status[1] while status[0] is zero.
+pbf_status launch_frame_done(uint32_t* status, void* stream);
 pbf_status launch_keys(const pbf_float4* p, uint32_t* key, size_t n, pbf_float3 mn, float cs,
                        pbf_int3 gs, const SceneTable& st, void* stream);
 // lambda goes to p[].w (the step's Delta-p reads it there).
 pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* cnt, int maxN,
                                  size_t n, float h, float mass, float rho0, float eps,
-                                 size_t stride, const uint32_t* status, void* stream);
+                                 size_t stride, const ListPass& lp, uint32_t* status, void* stream);
 // Reads lambda from pin[].w and restores pout[].w from x[].w.
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
                                       const int* cnt, int maxN, const pbf_float4* x, size_t n,
                                       float h, float mass, float rho0, float sk, int sn, float sdq,
                                       size_t stride, const pbfdev::Colliders& col, float radius,
-                                      const SceneTable& st, uint32_t* status, void* stream);
+                                      const SceneTable& st, const ListPass& lp, uint32_t* status,
+                                      void* stream);
 pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pbf_float4* vold,
                               pbf_float4* vout, const uint32_t* nbr, const int* cnt, int maxN,
                               size_t n, float h, float visc, float inv_dt, size_t stride,
-                              const uint32_t* status, void* stream);
+                              const ListPass& lp, uint32_t* status, void* stream);
 pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
                                        size_t n, float inv_dt, const uint32_t* status,
                                        void* stream);
 // Writes omega (with |omega| in .w) and the records pw = (p.xyz, |omega|).
+// lp.exactMax > 0: also counts the neighbours within h (the post-solve
+// build's overflow check when XSPH is off).
 pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint32_t* nbr,
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
-                            pbf_float4* pw, size_t stride, void* stream);
+                            pbf_float4* pw, size_t stride, const ListPass& lp, uint32_t* status,
+                            void* stream);
 pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,
                                   const int* cnt, int maxN, const pbf_float4* omega,
                                   const pbf_float4* vcur,
                                   const pbf_float4* vsnap, pbf_float4* vout, size_t n, float h,
                                   float dt, float strength, size_t stride,
                                   const pbfdev::Colliders& col, float radius, float e, float f,
-                                  const SceneTable& st, const uint32_t* status, void* stream);
+                                  const SceneTable& st, const ListPass& lp,
+                                  const uint32_t* status, void* stream);
 pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,
                                   size_t n, const pbfdev::Colliders& col, float r, float e,
                                   float f, const SceneTable& st, const uint32_t* status,
@@ -161,7 +194,7 @@ pbf_status tape_lists(const uint32_t* src, uint32_t* dst, const int* cnt, size_t
                       int maxGroups, void* stream);
 
 // Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
-// have cell size == h (a 3x3x3 stencil) and the working set must be in the
+// have cell size >= h (a 3x3x3 stencil) and the working set must be in the
 // order of sortedKeys; tileEnd comes from launch_cell_ranges.
 struct SweepGrid {
     const int* cellStart;
@@ -173,6 +206,13 @@ struct SweepGrid {
     float h;
     int maxNeighbors;
     SceneTable st;  // a batched step's scenes
+    // Skin lists: positions are taken at their container points (box, or
+    // the scene table's containers, shrunk by radius), and a list longer
+    // than maxNeighbors raises PBF_STATUS_LIST_STALE instead of the
+    // overflow bit.
+    bool skin = false;
+    const pbf_container* box = nullptr;
+    float radius = 0.0f;
 };
 // Entries of the tileEnd array for a grid of these dims.
 size_t sweep_tile_count(pbf_int3 dims);

@@ -25,17 +25,18 @@ import numpy as np
 
 __all__ = [
     "lib", "LIB_PATH", "float3", "int3", "SimulationParams", "Container", "SphereCollider",
-    "BoxCollider", "PlaneCollider", "make_float3", "DeviceArray", "PBFSolver", "SpatialGrid",
-    "PBFBatchSolver", "PBFAdjoint", "pinned_empty",
+    "BoxCollider", "PlaneCollider", "make_float3", "Tuning", "DeviceArray", "PBFSolver",
+    "SpatialGrid", "PBFBatchSolver", "PBFAdjoint", "pinned_empty",
     "CollisionSystem", "kernels", "PbfError", "InvalidArgument", "LogicError", "OutOfRange",
     "LengthError", "NeighborOverflow", "CollisionNonConvergence", "DeviceError",
-    "STATUS_NEIGHBOR_OVERFLOW", "STATUS_COLLISION_NONCONVERGENCE",
+    "STATUS_NEIGHBOR_OVERFLOW", "STATUS_COLLISION_NONCONVERGENCE", "STATUS_LIST_STALE",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libpbf_amd.so"
 
 STATUS_NEIGHBOR_OVERFLOW = 0x1
 STATUS_COLLISION_NONCONVERGENCE = 0x2
+STATUS_LIST_STALE = 0x4
 
 
 class PbfError(Exception):
@@ -246,6 +247,11 @@ _SIGNATURES = {
     "pbf_solver_stream": (_vp, [_vp]),
     "pbf_solver_set_profiling": (_st, [_vp, _i]),
     "pbf_solver_set_graphs": (_st, [_vp, _i]),
+    "pbf_solver_set_neighbor_skin": (_st, [_vp, C.c_float]),
+    "pbf_solver_neighbor_skin": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(_sz)]),
+    "pbf_solver_set_tuning": (_st, [_vp, _vp]),
+    "pbf_solver_get_tuning": (_st, [_vp, _vp]),
+    "pbf_solver_autotune": (_st, [_vp, _sz, _vp, _sz, _vp, _vp]),
     "pbf_solver_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
     "pbf_solver_neighbor_lists": (_st, [_vp, _vp, _vp, _vp, _sz, _i]),
@@ -264,6 +270,10 @@ _SIGNATURES = {
     "pbf_batch_copy_velocities_to_host": (_st, [_vp, _vp, _sz]),
     "pbf_batch_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
     "pbf_batch_set_graphs": (_st, [_vp, _i]),
+    "pbf_batch_set_neighbor_skin": (_st, [_vp, C.c_float]),
+    "pbf_batch_stale_frames": (_st, [_vp, C.POINTER(_sz)]),
+    "pbf_batch_set_tuning": (_st, [_vp, _vp]),
+    "pbf_batch_autotune": (_st, [_vp, _sz, _vp, _sz, _vp, _vp]),
     "pbf_batch_set_profiling": (_st, [_vp, _i]),
     "pbf_batch_kernel_times": (_st, [_vp, C.c_char_p, _sz, C.POINTER(C.c_double),
                                      C.POINTER(C.c_uint64), _sz, C.POINTER(_sz)]),
@@ -324,6 +334,31 @@ def as_float4(a, n=None) -> np.ndarray:
 # --------------------------------------------------------------------------
 # Pinned host memory
 # --------------------------------------------------------------------------
+class Tuning(C.Structure):
+    """pbf_tuning: the skin (cell-size multiplier 1 + neighborSkin) and the
+    workgroup size of the neighbour passes."""
+    _fields_ = [("neighborSkin", C.c_float), ("passBlock", C.c_int)]
+
+    def __repr__(self):
+        return f"Tuning(neighborSkin={self.neighborSkin:.3g}, passBlock={self.passBlock})"
+
+
+DEFAULT_TUNING_COUNT = 12
+
+
+def _autotune(fn, handle, frames, candidates):
+    if candidates:
+        arr = (Tuning * len(candidates))(*[Tuning(float(a), int(b)) for a, b in candidates])
+        count, ptr = len(candidates), C.cast(arr, _vp)
+    else:
+        count, ptr = 0, None
+    ms = (C.c_double * max(count, DEFAULT_TUNING_COUNT))()
+    chosen = Tuning()
+    _check(fn(handle, frames, ptr, count, C.cast(ms, _vp), C.cast(C.pointer(chosen), _vp)))
+    tried = candidates or [(s, b) for s in (0.0, 0.1, 0.2, 0.3) for b in (64, 128, 256)]
+    return chosen, [(float(a), int(b), ms[k]) for k, (a, b) in enumerate(tried)]
+
+
 class _PinnedBlock:
     def __init__(self, nbytes: int):
         p = _vp()
@@ -638,6 +673,37 @@ class PBFSolver:
         """Replay frames from captured HIP graphs, or launch eagerly (default)."""
         _check(lib().pbf_solver_set_graphs(self._h, 1 if enabled else 0))
 
+    def setNeighborSkin(self, skin: float):
+        """Skin lists: one grid build per substep, radius h * (1 + skin); 0 runs
+        the reference's build before every iteration and after the solve."""
+        _check(lib().pbf_solver_set_neighbor_skin(self._h, float(skin)))
+
+    def neighborSkin(self) -> float:
+        sk, st = C.c_float(), _sz()
+        _check(lib().pbf_solver_neighbor_skin(self._h, C.byref(sk), C.byref(st)))
+        return sk.value
+
+    def staleFrames(self) -> int:
+        """Frames run again with the reference's builds (stale skin lists)."""
+        sk, st = C.c_float(), _sz()
+        _check(lib().pbf_solver_neighbor_skin(self._h, C.byref(sk), C.byref(st)))
+        return st.value
+
+    def setTuning(self, neighborSkin: float, passBlock: int = 256):
+        t = Tuning(float(neighborSkin), int(passBlock))
+        _check(lib().pbf_solver_set_tuning(self._h, C.cast(C.pointer(t), _vp)))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().pbf_solver_get_tuning(self._h, C.cast(C.pointer(t), _vp)))
+        return t
+
+    def autotune(self, frames: int, candidates=None):
+        """Times `frames` frames of each (skin, passBlock) candidate (default
+        set when None) from the current state and keeps the fastest.
+        Returns (chosen Tuning, [(skin, passBlock, ms per frame), ...])."""
+        return _autotune(lib().pbf_solver_autotune, self._h, frames, candidates)
+
     def setProfiling(self, enabled: bool):
         _check(lib().pbf_solver_set_profiling(self._h, 1 if enabled else 0))
 
@@ -726,6 +792,21 @@ class PBFBatchSolver:
 
     def setGraphs(self, enabled: bool):
         _check(lib().pbf_batch_set_graphs(self._h, 1 if enabled else 0))
+
+    def setNeighborSkin(self, skin: float):
+        _check(lib().pbf_batch_set_neighbor_skin(self._h, float(skin)))
+
+    def staleFrames(self) -> int:
+        st = _sz()
+        _check(lib().pbf_batch_stale_frames(self._h, C.byref(st)))
+        return st.value
+
+    def setTuning(self, neighborSkin: float, passBlock: int = 256):
+        t = Tuning(float(neighborSkin), int(passBlock))
+        _check(lib().pbf_batch_set_tuning(self._h, C.cast(C.pointer(t), _vp)))
+
+    def autotune(self, frames: int, candidates=None):
+        return _autotune(lib().pbf_batch_autotune, self._h, frames, candidates)
 
     def setProfiling(self, enabled: bool):
         _check(lib().pbf_batch_set_profiling(self._h, 1 if enabled else 0))

@@ -53,6 +53,16 @@ public:
     std::size_t sceneFirst(std::size_t scene) const;
 
     void setGraphs(bool enabled);
+    // Skin lists for every scene (PBFSolver::setNeighborSkin).  A frame in
+    // which any scene's lists go stale runs again for every scene with the
+    // reference's builds.
+    void setNeighborSkin(float skin);
+    std::size_t staleFrames() const;
+    // Launch tunings and the autotuner of PBFSolver, over the whole batch.
+    void setTuning(const PBFSolver::Tuning& tuning);
+    PBFSolver::Tuning tuning() const;
+    std::vector<PBFSolver::TuningTime> autotune(
+        std::size_t frames, const std::vector<PBFSolver::Tuning>& candidates = {});
     void setProfiling(bool enabled);
     std::vector<PBFSolver::KernelTime> kernelTimes() const;
 

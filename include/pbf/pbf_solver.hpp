@@ -58,6 +58,39 @@ public:
     // Replay frames from captured HIP graphs (default off; see
     // pbf_solver_set_graphs).  No counterpart in the reference.
     void setGraphs(bool enabled);
+    // Skin lists: one grid build per substep instead of one per constraint
+    // iteration plus the post-solve rebuild (see
+    // pbf_solver_set_neighbor_skin).  skin is a fraction of the smoothing
+    // radius in [0, 1]; 0 runs the reference's builds.  No counterpart in
+    // the reference.
+    void setNeighborSkin(float skin);
+    float neighborSkin() const;
+    // Frames run again with the reference's builds because their skin
+    // lists went stale, since initialize.
+    std::size_t staleFrames() const;
+
+    // Launch tunings, no counterpart in the reference: the skin (the grid's
+    // cell-size multiplier is 1 + neighborSkin) and the workgroup size of
+    // the neighbour passes (64, 128, 192 or 256 threads; results do not
+    // depend on it).
+    struct Tuning {
+        float neighborSkin = 0.0f;
+        int passBlock = 256;
+    };
+    struct TuningTime {
+        Tuning tuning;
+        double msPerFrame;
+    };
+    void setTuning(const Tuning& tuning);
+    Tuning tuning() const;
+    // The autotuner: runs `frames` frames of every candidate (the default
+    // set when empty: skins 0, 0.1, 0.2, 0.3 by workgroups of 64, 128,
+    // 256) from the current state, restoring it after each, times them on
+    // the host clock (stale frames run again included), and keeps the
+    // fastest.  The state, and staleFrames(), are left as they were.
+    std::vector<TuningTime> autotune(std::size_t frames,
+                                     const std::vector<Tuning>& candidates = {});
+    static std::vector<Tuning> defaultTunings();
     struct KernelTime {
         std::string name;
         double milliseconds;

@@ -89,6 +89,10 @@ typedef enum pbf_status {
  * reference's per-phase host synchronisations. */
 #define PBF_STATUS_NEIGHBOR_OVERFLOW 0x1u
 #define PBF_STATUS_COLLISION_NONCONVERGENCE 0x2u
+/* A particle moved farther than half the skin of the step's skin lists
+ * (pbf_solver_set_neighbor_skin), or a skin list ran out of room: the
+ * frame is run again with a grid build per iteration.  Never an error. */
+#define PBF_STATUS_LIST_STALE 0x4u
 
 /* Thread-local message for the last failing call on this thread. */
 const char* pbf_last_error(void);
@@ -369,6 +373,48 @@ void* pbf_solver_stream(pbf_solver* solver);
  * eagerly while profiling is on. */
 pbf_status pbf_solver_set_graphs(pbf_solver* solver, int enabled);
 
+/* Skin lists (Verlet lists), no counterpart in the reference.  With skin > 0
+ * a substep builds the grid and the neighbour lists once, instead of before
+ * every constraint iteration and after the solve (pbf_solver.cu:210-282):
+ * around the container points of the predicted positions, with radius
+ * h * (1 + skin) on cells of that size.  Every pass still tests each entry
+ * against h with the reference's FP32 test, so it sums the same neighbours
+ * (in the skin list's order), and the counts within h raise the reference's
+ * overflow_error.  The lists are complete while no particle has moved more
+ * than 0.45 * skin * h from its container point: a Delta-p pass that sees a
+ * particle past that marks the lists stale, and the frame runs again from a
+ * checkpoint taken at its start with the reference's builds (counted by
+ * staleFrames).  skin lies in [0, 1]; 0 (the default) runs the reference's
+ * builds.  Results equal the reference's within the FP32 rounding of the
+ * summation order. */
+pbf_status pbf_solver_set_neighbor_skin(pbf_solver* solver, float skin);
+/* The skin, and the frames run again since initialize because their skin
+ * lists went stale (either pointer may be null). */
+pbf_status pbf_solver_neighbor_skin(const pbf_solver* solver, float* skin, size_t* staleFrames);
+
+/* Launch tunings and the autotuner, no counterpart in the reference.  A
+ * tuning is the skin (the grid's cell-size multiplier is 1 + neighborSkin,
+ * see pbf_solver_set_neighbor_skin) and the workgroup size of the
+ * neighbour passes (64, 128, 192 or 256 threads; results do not depend on
+ * it). */
+typedef struct pbf_tuning {
+    float neighborSkin;
+    int passBlock;
+} pbf_tuning;
+/* Size of the default candidate set: skins 0, 0.1, 0.2, 0.3 by workgroups
+ * of 64, 128 and 256 threads. */
+#define PBF_DEFAULT_TUNING_COUNT 12
+pbf_status pbf_solver_set_tuning(pbf_solver* solver, const pbf_tuning* tuning);
+pbf_status pbf_solver_get_tuning(const pbf_solver* solver, pbf_tuning* tuning);
+/* Runs `frames` frames of each candidate (candidates == NULL: the default
+ * set) from the current state, restoring it after each; times them on the
+ * host clock, stale frames run again included; applies the fastest, writes
+ * it to *chosen and each candidate's milliseconds per frame to msPerFrame
+ * (count entries, or PBF_DEFAULT_TUNING_COUNT for the default set; either
+ * pointer may be null).  The particle state is left as it was. */
+pbf_status pbf_solver_autotune(pbf_solver* solver, size_t frames, const pbf_tuning* candidates,
+                               size_t count, double* msPerFrame, pbf_tuning* chosen);
+
 /* Per-kernel timing on the solver's stream with HIP events.  While enabled,
  * every launch of the step is bracketed by events; pbf_solver_kernel_times
  * returns accumulated milliseconds and launch counts per kernel name.
@@ -421,6 +467,15 @@ pbf_status pbf_batch_copy_velocities_to_host(pbf_batch* batch, pbf_float4* h_out
                                              size_t particleCount);
 pbf_status pbf_batch_counts(const pbf_batch* batch, size_t* sceneCount, size_t* particleCount);
 pbf_status pbf_batch_set_graphs(pbf_batch* batch, int enabled);
+/* Skin lists for every scene of the batch (pbf_solver_set_neighbor_skin).
+ * A frame in which any scene's lists go stale runs again for all scenes
+ * with the reference's builds. */
+pbf_status pbf_batch_set_neighbor_skin(pbf_batch* batch, float skin);
+pbf_status pbf_batch_stale_frames(const pbf_batch* batch, size_t* staleFrames);
+/* Tunings and the autotuner over the whole batch (pbf_solver_autotune). */
+pbf_status pbf_batch_set_tuning(pbf_batch* batch, const pbf_tuning* tuning);
+pbf_status pbf_batch_autotune(pbf_batch* batch, size_t frames, const pbf_tuning* candidates,
+                              size_t count, double* msPerFrame, pbf_tuning* chosen);
 pbf_status pbf_batch_set_profiling(pbf_batch* batch, int enabled);
 pbf_status pbf_batch_kernel_times(pbf_batch* batch, char* names, size_t namesBytes,
                                   double* milliseconds, uint64_t* launches, size_t capacity,

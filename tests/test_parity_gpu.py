@@ -83,6 +83,27 @@ def test_grid_bitexact_random_with_clamping(gpu, oracle64):
     assert np.array_equal(cen, oce)
 
 
+@pytest.mark.parametrize("cells_axis", [(6, 5, 4), (40, 30, 20), (160, 100, 64), (400, 300, 200)])
+def test_grid_bitexact_every_pass_count(gpu, oracle64, cells_axis):
+    """One to four radix passes (7, 15, 20 and 25 key bits), 300K keys in
+    293 tiles, ten builds in a row on one grid: the sort's tile tickets,
+    histograms and look-back words are reused from build to build, and no
+    pass reads the array it writes."""
+    rng = np.random.default_rng(sum(cells_axis))
+    n = 300_001
+    mx = tuple(0.1 * c for c in cells_axis)
+    g = gpu.grid(n, (0.0, 0.0, 0.0), mx, 0.1)
+    for rep in range(10):
+        p = np.empty((n, 4), np.float32)
+        for a in range(3):
+            p[:, a] = rng.uniform(-0.05, mx[a] + 0.05, n)
+        p[:, 3] = 1.0
+        sk, si, cst, cen = g.build(p)
+        osk, osi, ocs, oce, _, _ = oracle64.grid_build(p, (0.0, 0.0, 0.0), mx, 0.1)
+        assert np.array_equal(sk, osk) and np.array_equal(si, osi), f"build {rep}"
+        assert np.array_equal(cst, ocs) and np.array_equal(cen, oce), f"build {rep}"
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_grid_bitexact_dam_break(gpu, oracle64, config):
     sc = scenes.dam_break(config)
