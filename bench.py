@@ -32,6 +32,10 @@ Prints one JSON line:
                  five list builds per substep, as the reference)
   b_alg_roofline the same with the compulsory-byte model of one build per
                  step and no stored lists (356 + 56 I + 16 ceil(b/8) + 8 C/N)
+  double_dam_break_8m  the 8,388,608-particle double dam break (3 substeps
+                 per frame) with the same tuning: particle-steps/s over 10
+                 frames after 3 warm-up frames, and its rooflines (--no-8m
+                 skips it)
   batched        the batched workload (256 seeded scenes of 2,048-200,000
                  particles, about 10M in all): one PBFBatchSolver against a
                  loop of PBFSolver.step() over one solver per scene, and the
@@ -250,6 +254,33 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
             "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
 
 
+def double_dam_break(pbf_amd, scenes, tuning, steps: int = 10, warmup: int = 3) -> dict:
+    """The 8,388,608-particle double dam break (two 128 x 256 x 128 blocks
+    at opposite walls, 3 substeps per frame) with the tuning chosen on the
+    1M workload: particle-steps/s over `steps` frames after `warmup`, and
+    the compulsory-byte roofline of that rate."""
+    sc = scenes.dam_break(3)
+    elapsed, _, _ = run_frames(pbf_amd, sc, warmup, steps, profile=False, tuning=tuning)
+    sub = sc.params.substeps
+    value = sc.n * steps * sub / elapsed
+    grid = pbf_amd.SpatialGrid()
+    grid.initialize(1, sc.minBounds, sc.maxBounds, sc.params.smoothingRadius)
+    cells = grid.numCells()
+    bits = max(0, (cells - 1).bit_length())
+    b_alg = 356 + 56 * sc.params.solverIterations + 16 * -(-bits // 8) + 8 * cells / sc.n
+    return {"workload": f"{sc.name}: {sc.n} particles, 4 iterations, {sub} substeps per "
+                        f"1/60 s frame, frames {warmup + 1}-{warmup + steps}",
+            "value": value, "unit": "particle-steps/s", "ms_per_step": elapsed * 1e3 / steps,
+            "b_alg_roofline": {"bytes_per_particle_step": b_alg,
+                               "achieved": b_alg * value / 1e9, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": b_alg * value / 1e9 / HBM_PEAK_GBS},
+            "step_roofline": {"bytes_per_particle_step":
+                              step_bytes_per_particle(sc.params.solverIterations, bits, cells, sc.n),
+                              "frac": step_bytes_per_particle(sc.params.solverIterations, bits,
+                                                              cells, sc.n) * value / 1e9
+                              / HBM_PEAK_GBS}}
+
+
 def adjoint_cost(pbf_amd, scenes, frames: int = 60, every: int = 10) -> dict:
     """The differentiable workload (65,536 particles, 60 frames, checkpoints
     every 10): forward() against backward() (the taped replay of every
@@ -312,6 +343,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (65,536 particles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-8m", action="store_true",
+                    help="skip the 8M double dam break measured after the 1M workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default tuning (the reference's builds, 256-thread passes)")
     ap.add_argument("--tune-frames", type=int, default=3,
@@ -389,6 +422,10 @@ def main() -> int:
     b_alg_roof = {"bytes_per_particle_step": b_alg, "achieved": b_alg * value / 1e9,
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_alg * value / 1e9 / HBM_PEAK_GBS}
 
+    headline_8m = None
+    if a.config == 2 and not a.no_8m:
+        headline_8m = double_dam_break(pbf_amd, scenes, tuning["chosen"] if tuning else None)
+
     batched = None
     if not a.no_batched:
         batched = batched_vs_looped(pbf_amd, scenes, a.batch_scenes, frames=3)
@@ -425,6 +462,7 @@ def main() -> int:
         "valu": valu,
         "step_roofline": step_roof,
         "b_alg_roofline": b_alg_roof,
+        "double_dam_break_8m": headline_8m,
         "batched": batched,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
