@@ -461,16 +461,30 @@ constexpr int kNbrBatch = 4;  // a group of the step's lists
 template <bool kGrouped, class Load, class Use>
 __device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ nbr, size_t stride,
                                                    size_t i, int c, Load load, Use use) {
+    if (kGrouped) {
+        // The next group's entries are loaded while this group's records
+        // are gathered and used: the list read (a stream from HBM) and the
+        // record gathers (L2) overlap instead of alternating.
+        // The last group is loaded again in place of the one past it, so the
+        // load is unconditional.
+        const uint4* groups = reinterpret_cast<const uint4*>(nbr) + i;
+        const int lastGroup = (c - 1) / 4;
+        uint4 g = c > 0 ? groups[0] : make_uint4(0u, 0u, 0u, 0u);
+        for (int q0 = 0; q0 < c; q0 += kNbrBatch) {
+            const uint32_t j[kNbrBatch] = {g.x, g.y, g.z, g.w};
+            decltype(load(0u)) r[kNbrBatch];
+#pragma unroll
+            for (int u = 0; u < kNbrBatch; ++u) r[u] = load(j[u]);
+            g = groups[static_cast<size_t>(min(q0 / 4 + 1, lastGroup)) * stride];
+#pragma unroll
+            for (int u = 0; u < kNbrBatch; ++u)
+                if (q0 + u < c) use(r[u]);
+        }
+        return;
+    }
     for (int q0 = 0; q0 < c; q0 += kNbrBatch) {
         uint32_t j[kNbrBatch];
-        if (kGrouped) {
-            const uint4* groups = reinterpret_cast<const uint4*>(nbr);
-            const uint4 g = groups[static_cast<size_t>(q0 / 4) * stride + i];
-            j[0] = g.x;
-            j[1] = g.y;
-            j[2] = g.z;
-            j[3] = g.w;
-        } else {
+        {
 #pragma unroll
             for (int u = 0; u < kNbrBatch; ++u)
                 j[u] = nbr[static_cast<size_t>(min(q0 + u, c - 1)) * stride + i];
