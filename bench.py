@@ -80,7 +80,7 @@ PHASE_BYTES = {
     "gather": 104,                     # read id + 3 records; write 3 records + id
     "build_lists": 20 + LIST,          # read p*, key; write the list
     "lambda": 20 + LIST,               # read p*, the list; write lambda
-    "delta_apply_collide": 36 + LIST,  # read p*, lambda, the list; write p'
+    "delta_apply_collide": 40 + LIST,  # read p*, lambda, the list; write p' and its key
     "update_xsph": 48 + LIST,          # read x, p*, the list; write v
     "update_velocity": 48,
     "vorticity": 48 + LIST,            # read p*, v, the list; write omega
@@ -108,12 +108,13 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
 def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) -> float:
     """Algorithmic bytes of one substep of this dataflow per particle: the
     reference's five grid builds (one per constraint iteration plus the
-    post-solve rebuild), each a key pass, a sort, the ranges, the permute
-    and the neighbour lists."""
+    post-solve rebuild), each a sort, the ranges, the permute and the
+    neighbour lists; the keys come from the prediction and the Delta-p
+    passes."""
     builds = iterations + 1
     build = (16 * ((key_bits + 7) // 8) + 4 + 8 * cells / n + PHASE_BYTES["gather"]
              + PHASE_BYTES["build_lists"])
-    return (PHASE_BYTES["predict_key"] + (builds - 1) * PHASE_BYTES["keys"] + builds * build
+    return (PHASE_BYTES["predict_key"] + builds * build
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
             + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
             + PHASE_BYTES["confine_resolve"])

@@ -10,7 +10,7 @@
 //                  neighbour lists                    -> k_build_lists (LDS cell sweep)
 //                  lambda (into p*.w)                 -> k_lambda_list
 //                  delta-p, apply, collision solve    -> k_delta_apply_collide
-//                  (restores p*.w from x.w)
+//                  (restores p*.w from x.w; writes the next build's keys)
 //   post-solve rebuild (if XSPH or vorticity)         -> as above
 //   velocity update + XSPH                            -> k_update_xsph
 //   vorticity (+ (x, |omega|) records)                -> k_vorticity_list
@@ -428,16 +428,12 @@ struct PBFSolver::State {
 
     // Grid build of the current p*, then the permutation of the working set
     // into cell order (SpatialGrid::build + reorderWorkingSet,
-    // pbf_solver.cu:154-172, spatial_grid.cu:139-226).  `keysReady` skips
-    // the key pass when the predict kernel already wrote the keys.
-    void build_and_reorder(bool keysReady, bool sk = false) {
+    // pbf_solver.cu:154-172, spatial_grid.cu:139-226).  The keys of p* are
+    // already in grid.keys: the prediction kernel writes them for a
+    // substep's first build, each Delta-p pass for the build after it.
+    void build_and_reorder(bool sk = false) {
         const std::size_t n = count;
         GridCore& grid = grid_of(sk);
-        if (!keysReady) {
-            Scope sc(prof, "keys", stream);
-            launch_check(pbfint::launch_keys(p.data(), grid.keys.data(), n, grid.minBounds,
-                                             grid.cellSize, grid.dims, scenes(), stream));
-        }
         {
             Scope sc(prof, "radix_sort", stream);
             launch_check(pbfint::radix_sort_pairs(grid.keys.data(), nullptr,
@@ -540,7 +536,7 @@ struct PBFSolver::State {
                     sgrid.dims, col.container, P.particleRadius, scenes(true), status.data(),
                     stream));
             }
-            build_and_reorder(true, true);
+            build_and_reorder(true);
             build_lists(true);
             for (int it = 0; it < P.solverIterations; ++it) {
                 sk.p0 = it == 0 ? p.data() : pq.data();
@@ -620,7 +616,7 @@ struct PBFSolver::State {
                     nullptr, 0.0f, scenes(), status.data(), stream));
             }
             for (int it = 0; it < P.solverIterations; ++it) {
-                build_and_reorder(it == 0);
+                build_and_reorder();
                 build_lists();
                 tape_build(it);
                 {
@@ -633,18 +629,22 @@ struct PBFSolver::State {
                 if (SubstepTape* tp = tape()) tape_copy(tp->pk[it], p, n);
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
+                    pbfint::ListPass dl = lpass();
+                    dl.keyOut = grid.keys.data();  // the next build's keys
+                    dl.keyMin = grid.minBounds;
+                    dl.keyCell = grid.cellSize;
+                    dl.keyDims = grid.dims;
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
-                        stride, col, P.particleRadius, scenes(), lpass(), status.data(),
-                        stream));
+                        stride, col, P.particleRadius, scenes(), dl, status.data(), stream));
                 }
                 if (SubstepTape* tp = tape()) tape_copy(tp->pout[it], pa, n);
                 std::swap(p, pa);
             }
             const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
             if (post) {
-                build_and_reorder(false);
+                build_and_reorder();
                 build_lists();
                 tape_build(4);
                 if (SubstepTape* tp = tape()) {

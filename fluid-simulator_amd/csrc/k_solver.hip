@@ -551,7 +551,8 @@ __global__ void __launch_bounds__(kBlock)
                           int maxN, const pbf_float4* __restrict__ x, size_t n, float h,
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
                           Colliders col, float radius, pbfint::SceneTable st,
-                          const pbf_float4* p0, float stale2, uint32_t* __restrict__ status) {
+                          const pbf_float4* p0, float stale2, uint32_t* __restrict__ key,
+                          pbf_float3 kmn, float kcs, pbf_int3 kgs, uint32_t* __restrict__ status) {
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -589,6 +590,7 @@ __global__ void __launch_bounds__(kBlock)
     if (!collision_fixed_point(pi, ci, radius))
         atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
     st4(pout, i, pi);
+    if (key != nullptr) key[i] = cell_key<kBatched>(pi, i, kmn, kcs, kgs, st);
     if (kSkin) {
         const float3 q = container_point(ld4(p0, i), *ci.container, radius);
         const float ex = pi.x - q.x, ey = pi.y - q.y, ez = pi.z - q.z;
@@ -936,10 +938,10 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
                                       void* stream) {
     const bool batched = st.geom != nullptr;
     const bool skinned = lp.p0 != nullptr;
-#define PBF_DELTA(B, S)                                                                        \
-    PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin, pout, nbr, cnt, \
-               maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, st, lp.p0,      \
-               lp.stale2, status)
+#define PBF_DELTA(B, S)                                                                     \
+    PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin,  \
+               pout, nbr, cnt, maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, \
+               st, lp.p0, lp.stale2, lp.keyOut, lp.keyMin, lp.keyCell, lp.keyDims, status)
     if (batched && skinned) PBF_DELTA(true, true);
     if (batched) PBF_DELTA(true, false);
     if (skinned) PBF_DELTA(false, true);

@@ -46,6 +46,13 @@ struct ListPass {
     float stale2 = 0.0f;             // squared displacement from their points that is too far
     int exactMax = 0;                // > 0: neighbours within h above this raise the overflow bit
     int block = 256;
+    // Delta-p: with keyOut set, also the cell keys of its output positions
+    // in this grid (the next grid build's keys; a batched step offsets them
+    // by the scene table).
+    uint32_t* keyOut = nullptr;
+    pbf_float3 keyMin{};
+    float keyCell = 0.0f;
+    pbf_int3 keyDims{};
 };
 
 // Grid (k_grid.hip)
