@@ -78,6 +78,7 @@ PHASE_BYTES = {
     "predict_key": 72,                 # read x, v; write p*, v_in, key, id
     "keys": 20,                        # read p*; write key
     "gather": 104,                     # read id + 3 records; write 3 records + id
+    "ranges_gather": 108,              # the gather, and the sorted key for the ranges
     "build_lists": 20 + LIST,          # read p*, key; write the list
     "lambda": 20 + LIST,               # read p*, the list; write lambda
     "delta_apply_collide": 40 + LIST,  # read p*, lambda, the list; write p' and its key
@@ -101,6 +102,8 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
         return 16.0 * ((key_bits + 7) // 8) * n
     if kernel == "cell_ranges":
         return 4.0 * n + 8.0 * cells
+    if kernel == "fill_cells":
+        return 4.0 * cells
     b = PHASE_BYTES.get(kernel)
     return None if b is None else float(b) * n
 
@@ -112,7 +115,7 @@ def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) 
     neighbour lists; the keys come from the prediction and the Delta-p
     passes."""
     builds = iterations + 1
-    build = (16 * ((key_bits + 7) // 8) + 4 + 8 * cells / n + PHASE_BYTES["gather"]
+    build = (16 * ((key_bits + 7) // 8) + 4 * cells / n + PHASE_BYTES["ranges_gather"]
              + PHASE_BYTES["build_lists"])
     return (PHASE_BYTES["predict_key"] + builds * build
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])

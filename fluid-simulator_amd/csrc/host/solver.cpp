@@ -6,7 +6,7 @@
 // stable ids map it back to API order on copy-out.  Per substep the device
 // runs (reference stage -> kernel here):
 //   integrate + v_in snapshot + first cell keys      -> k_predict_key
-//   per iteration: grid build, reorder               -> radix sort, ranges, k_gather
+//   per iteration: grid build, reorder               -> radix sort, k_ranges_gather
 //                  neighbour lists                    -> k_build_lists (LDS cell sweep)
 //                  lambda (into p*.w)                 -> k_lambda_list
 //                  delta-p, apply, collision solve    -> k_delta_apply_collide
@@ -443,19 +443,21 @@ struct PBFSolver::State {
                                                   grid.scratch.data(), stream));
         }
         {
-            Scope sc(prof, "cell_ranges", stream);
-            launch_check(pbfint::launch_fill_cells(grid.cellStart.data(), grid.cellEnd.data(),
+            Scope sc(prof, "fill_cells", stream);
+            // The step's readers (the list build) look at cellEnd only where
+            // cellStart is set, so only cellStart is cleared.
+            launch_check(pbfint::launch_fill_cells(grid.cellStart.data(), nullptr,
                                                    grid.numCells, stream));
-            launch_check(pbfint::launch_cell_ranges(grid.sortedKeys.data(), grid.cellStart.data(),
-                                                    grid.cellEnd.data(), n, grid.dims,
-                                                    tiles_of(sk), scenes(sk), stream));
         }
         {
-            Scope sc(prof, "gather", stream);
-            launch_check(pbfint::launch_gather(grid.sortedIndices.data(), x.data(), p.data(),
-                                               v.data(), vin.data(), id.data(), xa.data(),
-                                               pa.data(), va.data(), vina.data(), ida.data(), n,
-                                               stream));
+            Scope sc(prof, "ranges_gather", stream);
+            const pbfint::Permute pm{grid.sortedIndices.data(), x.data(), p.data(), v.data(),
+                                     vin.data(), id.data(), xa.data(), pa.data(), va.data(),
+                                     vina.data(), ida.data()};
+            launch_check(pbfint::launch_ranges_gather(grid.sortedKeys.data(),
+                                                      grid.cellStart.data(), grid.cellEnd.data(),
+                                                      n, grid.dims, tiles_of(sk), scenes(sk), pm,
+                                                      stream));
         }
         std::swap(x, xa);
         std::swap(p, pa);

@@ -254,14 +254,15 @@ __global__ void __launch_bounds__(kBlock) k_sort_identity(const uint32_t* __rest
     vout[i] = static_cast<uint32_t>(i);
 }
 
-// cudaMemset(-1) of cellStart and cellEnd (spatial_grid.cu:203-204).
+// cudaMemset(-1) of cellStart and cellEnd (spatial_grid.cu:203-204); ce may
+// be null when only cellStart has readers of empty cells.
 __global__ void __launch_bounds__(kBlock) k_fill_cells(int* __restrict__ cs, int* __restrict__ ce,
                                                        size_t cells) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < cells;
          i += stride) {
         cs[i] = -1;
-        ce[i] = -1;
+        if (ce != nullptr) ce[i] = -1;
     }
 }
 
@@ -302,6 +303,33 @@ __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restri
         if (i == n - 1 || sweep_tile_of<kBatched>(sk[i + 1], i + 1, gx, chunks, st) != t)
             tileEnd[t] = static_cast<uint32_t>(i + 1);
     }
+}
+
+// The cell ranges of the sorted keys (k_cell_ranges) and, in the same
+// pass, the permutation of the working set into that order
+// (gatherSolverState, src/pbf/pbf_solver_kernels.cu:387-407): slot i of the
+// outputs takes the records of slot idx[i].
+template <bool kBatched>
+__global__ void __launch_bounds__(kBlock)
+    k_ranges_gather(const uint32_t* __restrict__ sk, int* __restrict__ cs, int* __restrict__ ce,
+                    size_t n, uint32_t gx, uint32_t chunks, uint32_t* __restrict__ tileEnd,
+                    pbfint::SceneTable st, pbfint::Permute pm) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = pm.idx[i];
+    st4(pm.xo, i, ld4(pm.x, s));
+    st4(pm.po, i, ld4(pm.p, s));
+    st4(pm.vo, i, ld4(pm.v, s));
+    st4(pm.vino, i, ld4(pm.vin, s));
+    pm.ido[i] = pm.id[s];
+    const uint32_t k = sk[i];
+    if (i == 0 || sk[i - 1] != k) cs[k] = static_cast<int>(i);
+    const bool lastInCell = i == n - 1 || sk[i + 1] != k;
+    if (!lastInCell) return;
+    ce[k] = static_cast<int>(i + 1);
+    const uint32_t t = sweep_tile_of<kBatched>(k, i, gx, chunks, st);
+    if (i == n - 1 || sweep_tile_of<kBatched>(sk[i + 1], i + 1, gx, chunks, st) != t)
+        tileEnd[t] = static_cast<uint32_t>(i + 1);
 }
 
 // findNeighbors (src/pbf/neighbors.cu:3-91).  Cells are swept z, y, x over
@@ -392,6 +420,19 @@ pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pb
                    tileEnd, st);
     PBF_LAUNCH(k_cell_ranges<false>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
                tileEnd, st);
+}
+
+pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
+                                uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
+                                void* stream) {
+    if (n == 0) return PBF_OK;
+    const uint32_t gx = static_cast<uint32_t>(dims.x);
+    const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
+    if (st.geom != nullptr)
+        PBF_LAUNCH(k_ranges_gather<true>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
+                   tileEnd, st, pm);
+    PBF_LAUNCH(k_ranges_gather<false>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
+               tileEnd, st, pm);
 }
 
 size_t radix_scratch_words(size_t n) {

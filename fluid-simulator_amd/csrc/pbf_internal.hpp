@@ -64,12 +64,26 @@ pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* 
                                  void* stream);
 // Cells along x of one neighbour-sweep tile.
 constexpr uint32_t kSweepTileW = 32;
+// -1 into cellStart and, unless ce is null, cellEnd.
 pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
 // Cell ranges of the sorted keys; with tileEnd not null, also the end of
 // every non-empty sweep tile (sweep_tile_count(dims) entries, only those of
 // non-empty tiles written).
 pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
                               uint32_t* tileEnd, const SceneTable& st, void* stream);
+// The working-set permutation of a grid build: outputs xo..ido at slot i
+// take the records of slot idx[i].
+struct Permute {
+    const uint32_t* idx;
+    const pbf_float4 *x, *p, *v, *vin;
+    const uint32_t* id;
+    pbf_float4 *xo, *po, *vo, *vino;
+    uint32_t* ido;
+};
+// launch_cell_ranges (tileEnd required) and the permutation in one pass.
+pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
+                                uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
+                                void* stream);
 size_t radix_scratch_words(size_t n);
 int radix_key_bits(size_t numCells);
 // Stable sort of (key, value) pairs by the low `bits` key bits into
