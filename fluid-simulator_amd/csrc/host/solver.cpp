@@ -429,8 +429,9 @@ struct PBFSolver::State {
     // Grid build of the current p*, then the permutation of the working set
     // into cell order (SpatialGrid::build + reorderWorkingSet,
     // pbf_solver.cu:154-172, spatial_grid.cu:139-226).  The keys of p* are
-    // already in grid.keys: the prediction kernel writes them for a
-    // substep's first build, each Delta-p pass for the build after it.
+    // already in grid.keys and cellStart is already cleared: the prediction
+    // kernel does both for a substep's first build, each Delta-p pass for
+    // the build after it.
     void build_and_reorder(bool sk = false) {
         const std::size_t n = count;
         GridCore& grid = grid_of(sk);
@@ -441,13 +442,6 @@ struct PBFSolver::State {
                                                   grid.sortedIndices.data(), grid.keysTmp.data(),
                                                   grid.indices.data(), n, grid.keyBits,
                                                   grid.scratch.data(), stream));
-        }
-        {
-            Scope sc(prof, "fill_cells", stream);
-            // The step's readers (the list build) look at cellEnd only where
-            // cellStart is set, so only cellStart is cleared.
-            launch_check(pbfint::launch_fill_cells(grid.cellStart.data(), nullptr,
-                                                   grid.numCells, stream));
         }
         {
             Scope sc(prof, "ranges_gather", stream);
@@ -535,8 +529,8 @@ struct PBFSolver::State {
                 launch_check(pbfint::launch_predict_key(
                     x.data(), p.data(), v.data(), vin.data(), sgrid.keys.data(),
                     static_cast<int>(n), sdt, P.gravity, sgrid.minBounds, sgrid.cellSize,
-                    sgrid.dims, col.container, P.particleRadius, scenes(true), status.data(),
-                    stream));
+                    sgrid.dims, col.container, P.particleRadius, scenes(true),
+                    sgrid.cellStart.data(), sgrid.numCells, status.data(), stream));
             }
             build_and_reorder(true);
             build_lists(true);
@@ -615,7 +609,8 @@ struct PBFSolver::State {
                 launch_check(pbfint::launch_predict_key(
                     x.data(), p.data(), v.data(), vin.data(), grid.keys.data(),
                     static_cast<int>(n), sdt, P.gravity, grid.minBounds, grid.cellSize, grid.dims,
-                    nullptr, 0.0f, scenes(), status.data(), stream));
+                    nullptr, 0.0f, scenes(), grid.cellStart.data(), grid.numCells, status.data(),
+                    stream));
             }
             for (int it = 0; it < P.solverIterations; ++it) {
                 build_and_reorder();
@@ -632,10 +627,12 @@ struct PBFSolver::State {
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
                     pbfint::ListPass dl = lpass();
-                    dl.keyOut = grid.keys.data();  // the next build's keys
+                    dl.keyOut = grid.keys.data();  // the next build's keys and cells
                     dl.keyMin = grid.minBounds;
                     dl.keyCell = grid.cellSize;
                     dl.keyDims = grid.dims;
+                    dl.clearCells = grid.cellStart.data();
+                    dl.clearCount = grid.numCells;
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,

@@ -37,6 +37,19 @@ __device__ __forceinline__ uint32_t cell_key(float4 p, size_t i, pbf_float3 mn, 
     return sg.cellOff + hash_key(p, sg.minBounds, cs, sg.dims);
 }
 
+// cudaMemset(-1) of the next grid build's cellStart (spatial_grid.cu:203-204),
+// folded into a kernel that runs between the previous build's list sweep
+// and the next build's ranges: thread t of the launch writes cells t,
+// t + threads, ...  Only cellStart is cleared: the list build reads a
+// cell's end only where its start is set.
+__device__ __forceinline__ void clear_cells(int* __restrict__ cells, size_t count) {
+    if (cells == nullptr) return;
+    const size_t threads = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < count;
+         c += threads)
+        cells[c] = -1;
+}
+
 // The colliders of slot i: in a batched step, its scene's container.
 template <bool kBatched>
 __device__ __forceinline__ Colliders colliders_of(const Colliders& col, size_t i,
@@ -479,7 +492,9 @@ __global__ void __launch_bounds__(kBlock)
                   pbf_float4* __restrict__ v, pbf_float4* __restrict__ vin,
                   uint32_t* __restrict__ key, int n, float dt, pbf_float3 g, pbf_float3 mn,
                   float cs, pbf_int3 gs, const pbf_container* __restrict__ box, float radius,
-                  pbfint::SceneTable st, const uint32_t* __restrict__ status) {
+                  pbfint::SceneTable st, int* __restrict__ clearCells, size_t clearCount,
+                  const uint32_t* __restrict__ status) {
+    clear_cells(clearCells, clearCount);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (*status != 0u) return;
@@ -552,7 +567,9 @@ __global__ void __launch_bounds__(kBlock)
                           float mass, float rho0, float sk, int sn, float sdq, size_t stride,
                           Colliders col, float radius, pbfint::SceneTable st,
                           const pbf_float4* p0, float stale2, uint32_t* __restrict__ key,
-                          pbf_float3 kmn, float kcs, pbf_int3 kgs, uint32_t* __restrict__ status) {
+                          pbf_float3 kmn, float kcs, pbf_int3 kgs, int* __restrict__ clearCells,
+                          size_t clearCount, uint32_t* __restrict__ status) {
+    clear_cells(clearCells, clearCount);
     const size_t i = thread_index<true>();
     if (i >= n) return;
     if (*status != 0u) return;
@@ -883,12 +900,13 @@ namespace pbfint {
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
                               float cs, pbf_int3 gs, const pbf_container* box, float radius,
-                              const SceneTable& st, const uint32_t* status, void* stream) {
+                              const SceneTable& st, int* clearCells, size_t clearCount,
+                              const uint32_t* status, void* stream) {
     if (st.geom != nullptr)
         PBF_LAUNCH(k_predict_key<true>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g,
-                   mn, cs, gs, box, radius, st, status);
+                   mn, cs, gs, box, radius, st, clearCells, clearCount, status);
     PBF_LAUNCH(k_predict_key<false>, grid_for(n), kBlock, stream, x, p, v, vin, key, n, dt, g, mn,
-               cs, gs, box, radius, st, status);
+               cs, gs, box, radius, st, clearCells, clearCount, status);
 }
 
 pbf_status launch_checkpoint(const pbf_float4* x, const pbf_float4* v, const uint32_t* id,
@@ -941,7 +959,8 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
 #define PBF_DELTA(B, S)                                                                     \
     PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin,  \
                pout, nbr, cnt, maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, \
-               st, lp.p0, lp.stale2, lp.keyOut, lp.keyMin, lp.keyCell, lp.keyDims, status)
+               st, lp.p0, lp.stale2, lp.keyOut, lp.keyMin, lp.keyCell, lp.keyDims,          \
+               lp.clearCells, lp.clearCount, status)
     if (batched && skinned) PBF_DELTA(true, true);
     if (batched) PBF_DELTA(true, false);
     if (skinned) PBF_DELTA(false, true);

@@ -53,6 +53,10 @@ struct ListPass {
     pbf_float3 keyMin{};
     float keyCell = 0.0f;
     pbf_int3 keyDims{};
+    // Delta-p: with clearCells set, -1 into the next build's cellStart
+    // (clearCount cells).
+    int* clearCells = nullptr;
+    size_t clearCount = 0;
 };
 
 // Grid (k_grid.hip)
@@ -96,11 +100,13 @@ pbf_status radix_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* 
 // Solver (k_solver.hip)
 // With box not null, the keys are those of the container points of p*
 // (skin lists); box is the single scene's container (a batched step uses
-// the scene table's).
+// the scene table's).  clearCells: -1 into the build's cellStart
+// (clearCount cells).
 pbf_status launch_predict_key(const pbf_float4* x, pbf_float4* p, pbf_float4* v, pbf_float4* vin,
                               uint32_t* key, int n, float dt, pbf_float3 g, pbf_float3 mn,
                               float cs, pbf_int3 gs, const pbf_container* box, float radius,
-                              const SceneTable& st, const uint32_t* status, void* stream);
+                              const SceneTable& st, int* clearCells, size_t clearCount,
+                              const uint32_t* status, void* stream);
 // Copies x, v and id to the frame checkpoint while *status is zero.
 pbf_status launch_checkpoint(const pbf_float4* x, const pbf_float4* v, const uint32_t* id,
                              pbf_float4* cx, pbf_float4* cv, uint32_t* cid, size_t n,
