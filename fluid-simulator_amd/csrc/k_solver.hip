@@ -1018,12 +1018,4 @@ pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, 
                st, status);
 }
 
-pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
-                         const pbf_float4* v, const pbf_float4* vin, const uint32_t* id,
-                         pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
-                         uint32_t* ido, size_t n, void* stream) {
-    PBF_LAUNCH(k_gather, grid_for(n), kBlock, stream, idx, x, p, v, vin, id, xo, po, vo, vino, ido,
-               n);
-}
-
 }  // namespace pbfint

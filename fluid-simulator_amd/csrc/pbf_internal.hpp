@@ -152,10 +152,6 @@ pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, 
                                   size_t n, const pbfdev::Colliders& col, float r, float e,
                                   float f, const SceneTable& st, const uint32_t* status,
                                   void* stream);
-pbf_status launch_gather(const uint32_t* idx, const pbf_float4* x, const pbf_float4* p,
-                         const pbf_float4* v, const pbf_float4* vin, const uint32_t* id,
-                         pbf_float4* xo, pbf_float4* po, pbf_float4* vo, pbf_float4* vino,
-                         uint32_t* ido, size_t n, void* stream);
 pbf_status launch_zero_u32(uint32_t* p, size_t n, void* stream);
 
 // Adjoint of the substep (k_adjoint.hip).  One grid build of the forward's
