@@ -53,6 +53,7 @@ constexpr int kThreads = 256;
 constexpr int kCap = 1536;             // staged particles per z-plane
 constexpr int kPad = 4;                // test reads run up to 3 records past a row
 constexpr int kMaxTilePasses = 4;      // more tiles per workgroup take the lane path
+constexpr int kStageUnroll = 3;        // staging loads in flight per thread
 
 struct Grid {
     const pbf_float4* p;
@@ -358,18 +359,33 @@ __device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& 
             const bool staged = pc <= kCap;
             if (staged) {
                 lds_barrier();  // the previous plane's readers are done
-                for (int rr = 0; rr < 3; ++rr) {
-                    const int s = 3 * pl + rr;
-                    if (m.segCells[s] == 0) continue;
-                    const int cnt = m.cellOff[s][m.segCells[s]];
-                    const size_t first = static_cast<size_t>(m.segFirst[s]);
-                    const int base = m.segBase[s];
-                    for (int q = tid; q < cnt; q += kThreads) {
-                        const float3 a = test_point<kSkin>(gm.box, g.radius, ld4(g.p, first + q));
-                        float* r = lds.xyz + 3 * (base + q);
-                        r[0] = a.x;
-                        r[1] = a.y;
-                        r[2] = a.z;
+                // The plane's three segments are consecutive in LDS: record q
+                // of [0, pc) is row 0 below segBase of row 1, row 1 below
+                // segBase of row 2, else row 2 (an empty row has no q).  A
+                // thread issues kStageUnroll loads before the first store.
+                const int s0 = 3 * pl;
+                const int b1 = m.segBase[s0 + 1];
+                const int b2 = m.segBase[s0 + 2];
+                const size_t f0 = static_cast<size_t>(m.segFirst[s0]);
+                const size_t f1 = static_cast<size_t>(m.segFirst[s0 + 1]) - b1;
+                const size_t f2 = static_cast<size_t>(m.segFirst[s0 + 2]) - b2;
+                for (int q0 = tid; q0 < pc; q0 += kStageUnroll * kThreads) {
+                    float4 v[kStageUnroll];
+#pragma unroll
+                    for (int u = 0; u < kStageUnroll; ++u) {
+                        const int q = q0 + u * kThreads;
+                        if (q < pc) v[u] = ld4(g.p, (q < b1 ? f0 : (q < b2 ? f1 : f2)) + q);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kStageUnroll; ++u) {
+                        const int q = q0 + u * kThreads;
+                        if (q < pc) {
+                            const float3 a = test_point<kSkin>(gm.box, g.radius, v[u]);
+                            float* r = lds.xyz + 3 * q;
+                            r[0] = a.x;
+                            r[1] = a.y;
+                            r[2] = a.z;
+                        }
                     }
                 }
                 lds_barrier();
