@@ -72,8 +72,8 @@ VALU_PEAK_TFLOPS = 78.6
 KMEAN, CAND = 29, 218  # neighbours and candidates per particle at rest
 LIST = 4 + 4 * KMEAN    # a neighbour list: its count and its entries
 # Algorithmic HBM bytes per particle of each step phase: float4 records,
-# 4-byte keys / ids / lambdas / list entries, neighbour records gathered on
-# chip (SURVEY.md's phase figures, plus the list the step stores and reads).
+# 4-byte keys / ids / lambdas / list entries read or written once, neighbour
+# records gathered on chip, plus the list the step stores and reads.
 PHASE_BYTES = {
     "predict_key": 72,                 # read x, v; write p*, v_in, key, id
     "keys": 20,                        # read p*; write key
@@ -88,8 +88,8 @@ PHASE_BYTES = {
     "confine_resolve": 80 + LIST,      # read 4 records, the list; write v
     "resolve_velocities": 64,
 }
-# Flop model: per neighbour of each pass (reference expression
-# trees) and 8 per candidate radius test.
+# Flop model: per neighbour of each pass (the FP32 operations of the
+# reference's expression trees) and 8 per candidate radius test.
 FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
                        "vorticity": 34, "confine_resolve": 32, "build_lists": 0}
 TESTING = {"build_lists"}  # passes that run the radius test
