@@ -78,7 +78,12 @@ PHASE_BYTES = {
     "predict_key": 72,                 # read x, v; write p*, v_in, key, id
     "keys": 20,                        # read p*; write key
     "gather": 104,                     # read id + 3 records; write 3 records + id
-    "ranges_gather": 108,              # the gather, and the sorted key for the ranges
+    # A build inside the constraint loop moves p* and composes the record
+    # permutation (read idx, p*, perm; write p*, perm; the sorted key for the
+    # ranges): 48.  The substep's last build moves every record through the
+    # composed permutation: 148.  Per launch, averaged over a substep's five
+    # builds at 4 iterations: (4 * 48 + 148) / 5 = 68.
+    "ranges_gather": 68,
     "build_lists": 20 + LIST,          # read p*, key; write the list
     "lambda": 20 + LIST,               # read p*, the list; write lambda
     "delta_apply_collide": 40 + LIST,  # read p*, lambda, the list; write p' and its key
@@ -88,6 +93,8 @@ PHASE_BYTES = {
     "confine_resolve": 80 + LIST,      # read 4 records, the list; write v
     "resolve_velocities": 64,
 }
+# The two kinds of build permutation (bytes per particle, as above).
+RANGES_GATHER_DEFERRED, RANGES_GATHER_FULL = 48, 148
 # Flop model: per neighbour of each pass (the FP32 operations of the
 # reference's expression trees) and 8 per candidate radius test.
 FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
@@ -115,9 +122,9 @@ def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) 
     neighbour lists; the keys come from the prediction and the Delta-p
     passes."""
     builds = iterations + 1
-    build = (16 * ((key_bits + 7) // 8) + 4 * cells / n + PHASE_BYTES["ranges_gather"]
-             + PHASE_BYTES["build_lists"])
-    return (PHASE_BYTES["predict_key"] + builds * build
+    build = 16 * ((key_bits + 7) // 8) + 4 * cells / n + PHASE_BYTES["build_lists"]
+    permute = RANGES_GATHER_DEFERRED * iterations + RANGES_GATHER_FULL
+    return (PHASE_BYTES["predict_key"] + builds * build + permute
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
             + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
             + PHASE_BYTES["confine_resolve"])

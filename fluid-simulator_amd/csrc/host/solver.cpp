@@ -244,6 +244,12 @@ struct PBFSolver::State {
     // status[0]: the status word; status[1]: frames finished before a
     // stale one.
     DeviceBuffer<std::uint32_t> id, ida, status;
+    // Grid builds inside the constraint loop permute p* alone (lambda and
+    // Delta-p read nothing else); x, v, v_in and id follow at the substep's
+    // last build.  While they lag, perm[i] is the slot of their records
+    // for p* slot i.
+    DeviceBuffer<std::uint32_t> perm, perma;
+    bool recordsLag = false;
     DeviceBuffer<std::uint32_t> nbr;      // grouped neighbour lists of the last build
     DeviceBuffer<int> cnt;
     std::size_t stride = 0;
@@ -305,7 +311,7 @@ struct PBFSolver::State {
     // with the permutation the frame applies; a workload cycles through one
     // or two of them.  Kernel arguments are baked in too: initialize,
     // setParticles and the collider setters drop the graphs.
-    static constexpr int kSlots = 11;
+    static constexpr int kSlots = 13;
     using BufferKey = std::array<const void*, kSlots>;
     struct FrameGraph {
         BufferKey start{};
@@ -346,22 +352,25 @@ struct PBFSolver::State {
         graphs.clear();
     }
 
-    // Slots 0-8: x, xa, p, pa, pq, v, va, vin, vina; 9-10: id, ida.
+    // Slots 0-8: x, xa, p, pa, pq, v, va, vin, vina; 9-12: id, ida, perm,
+    // perma.
     static constexpr int kF4Slots = 9;
+    static constexpr int kU32Slots = kSlots - kF4Slots;
     BufferKey buffer_key() const {
-        return {x.data(), xa.data(), p.data(), pa.data(), pq.data(), v.data(), va.data(),
-                vin.data(), vina.data(), id.data(), ida.data()};
+        return {x.data(),  xa.data(),  p.data(),   pa.data(),  pq.data(),
+                v.data(),  va.data(),  vin.data(), vina.data(), id.data(),
+                ida.data(), perm.data(), perma.data()};
     }
 
-    void permute(const std::array<int, kSlots>& perm) {
+    void permute(const std::array<int, kSlots>& to) {
         std::array<DeviceBuffer<float4>*, kF4Slots> f{&x, &xa, &p, &pa, &pq, &v, &va, &vin, &vina};
-        std::array<DeviceBuffer<std::uint32_t>*, 2> u{&id, &ida};
+        std::array<DeviceBuffer<std::uint32_t>*, kU32Slots> u{&id, &ida, &perm, &perma};
         std::array<DeviceBuffer<float4>, kF4Slots> tf;
-        std::array<DeviceBuffer<std::uint32_t>, 2> tu;
-        for (int k = 0; k < kF4Slots; ++k) tf[k] = std::move(*f[perm[k]]);
-        for (int k = 0; k < 2; ++k) tu[k] = std::move(*u[perm[kF4Slots + k] - kF4Slots]);
+        std::array<DeviceBuffer<std::uint32_t>, kU32Slots> tu;
+        for (int k = 0; k < kF4Slots; ++k) tf[k] = std::move(*f[to[k]]);
+        for (int k = 0; k < kU32Slots; ++k) tu[k] = std::move(*u[to[kF4Slots + k] - kF4Slots]);
         for (int k = 0; k < kF4Slots; ++k) *f[k] = std::move(tf[k]);
-        for (int k = 0; k < 2; ++k) *u[k] = std::move(tu[k]);
+        for (int k = 0; k < kU32Slots; ++k) *u[k] = std::move(tu[k]);
     }
 
     // The permutation that takes assignment `from` to assignment `to`.
@@ -432,7 +441,7 @@ struct PBFSolver::State {
     // already in grid.keys and cellStart is already cleared: the prediction
     // kernel does both for a substep's first build, each Delta-p pass for
     // the build after it.
-    void build_and_reorder(bool sk = false) {
+    void build_and_reorder(bool sk = false, bool deferRecords = false) {
         const std::size_t n = count;
         GridCore& grid = grid_of(sk);
         {
@@ -445,19 +454,33 @@ struct PBFSolver::State {
         }
         {
             Scope sc(prof, "ranges_gather", stream);
-            const pbfint::Permute pm{grid.sortedIndices.data(), x.data(), p.data(), v.data(),
-                                     vin.data(), id.data(), xa.data(), pa.data(), va.data(),
-                                     vina.data(), ida.data()};
+            pbfint::Permute pm{grid.sortedIndices.data(), x.data(), p.data(), v.data(),
+                               vin.data(), id.data(), xa.data(), pa.data(), va.data(),
+                               vina.data(), ida.data()};
+            pm.permIn = recordsLag ? perm.data() : nullptr;
+            if (deferRecords) {
+                pm.x = pm.v = pm.vin = nullptr;
+                pm.id = nullptr;
+                pm.xo = pm.vo = pm.vino = nullptr;
+                pm.ido = nullptr;
+                pm.permOut = perma.data();
+            }
             launch_check(pbfint::launch_ranges_gather(grid.sortedKeys.data(),
                                                       grid.cellStart.data(), grid.cellEnd.data(),
                                                       n, grid.dims, tiles_of(sk), scenes(sk), pm,
                                                       stream));
         }
-        std::swap(x, xa);
         std::swap(p, pa);
+        if (deferRecords) {
+            std::swap(perm, perma);
+            recordsLag = true;
+            return;
+        }
+        std::swap(x, xa);
         std::swap(v, va);
         std::swap(vin, vina);
         std::swap(id, ida);
+        recordsLag = false;
     }
 
     pbfint::SweepGrid sweep_grid(bool sk = false) {
@@ -603,6 +626,8 @@ struct PBFSolver::State {
         const float invDt = 1.0f / sdt;
         const float h = P.smoothingRadius;
         const pbfdev::Colliders col = collision.state().view();
+        const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
+        recordsLag = false;
         for (int substep = 0; substep < P.substeps; ++substep) {
             {
                 Scope sc(prof, "predict_key", stream);
@@ -613,7 +638,10 @@ struct PBFSolver::State {
                     stream));
             }
             for (int it = 0; it < P.solverIterations; ++it) {
-                build_and_reorder();
+                // The substep's last build moves every record; the tapes
+                // record ids at every build.
+                const bool last = !post && it == P.solverIterations - 1;
+                build_and_reorder(false, !last && tapes == nullptr);
                 build_lists();
                 tape_build(it);
                 {
@@ -633,6 +661,7 @@ struct PBFSolver::State {
                     dl.keyDims = grid.dims;
                     dl.clearCells = grid.cellStart.data();
                     dl.clearCount = grid.numCells;
+                    dl.xperm = recordsLag ? perm.data() : nullptr;
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
@@ -641,7 +670,6 @@ struct PBFSolver::State {
                 if (SubstepTape* tp = tape()) tape_copy(tp->pout[it], pa, n);
                 std::swap(p, pa);
             }
-            const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
             if (post) {
                 build_and_reorder();
                 build_lists();
@@ -955,6 +983,8 @@ void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_, 
         b->allocate(maxParticles_);
     id.allocate(maxParticles_);
     ida.allocate(maxParticles_);
+    perm.allocate(maxParticles_);
+    perma.allocate(maxParticles_);
     ckid.allocate(maxParticles_);
     status.allocate(2);
     nbr.allocate(stride_ * static_cast<std::size_t>(kMaxNeighbors));

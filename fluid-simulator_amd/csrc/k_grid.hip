@@ -317,11 +317,16 @@ __global__ void __launch_bounds__(kBlock)
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = pm.idx[i];
-    st4(pm.xo, i, ld4(pm.x, s));
     st4(pm.po, i, ld4(pm.p, s));
-    st4(pm.vo, i, ld4(pm.v, s));
-    st4(pm.vino, i, ld4(pm.vin, s));
-    pm.ido[i] = pm.id[s];
+    const uint32_t r = pm.permIn != nullptr ? pm.permIn[s] : s;
+    if (pm.x != nullptr) {
+        st4(pm.xo, i, ld4(pm.x, r));
+        st4(pm.vo, i, ld4(pm.v, r));
+        st4(pm.vino, i, ld4(pm.vin, r));
+        pm.ido[i] = pm.id[r];
+    } else {
+        pm.permOut[i] = r;
+    }
     const uint32_t k = sk[i];
     if (i == 0 || sk[i - 1] != k) cs[k] = static_cast<int>(i);
     const bool lastInCell = i == n - 1 || sk[i + 1] != k;
@@ -426,6 +431,10 @@ pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, 
                                 uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
                                 void* stream) {
     if (n == 0) return PBF_OK;
+    if (pm.x == nullptr && pm.permOut == nullptr)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "a p*-only permutation needs permOut");
+    if (pm.permOut != nullptr && pm.permOut == pm.permIn)
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "permutation composed in place");
     const uint32_t gx = static_cast<uint32_t>(dims.x);
     const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
     if (st.geom != nullptr)

@@ -57,6 +57,9 @@ struct ListPass {
     // (clearCount cells).
     int* clearCells = nullptr;
     size_t clearCount = 0;
+    // Delta-p: with xperm set, the record x of slot i is at xperm[i] (the
+    // grid builds since the last full permutation moved p* only).
+    const uint32_t* xperm = nullptr;
 };
 
 // Grid (k_grid.hip)
@@ -77,12 +80,19 @@ pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pb
                               uint32_t* tileEnd, const SceneTable& st, void* stream);
 // The working-set permutation of a grid build: outputs xo..ido at slot i
 // take the records of slot idx[i].
+// The working-set permutation of a grid build: slot i takes the records of
+// slot idx[i].  With x null only p* moves, and permOut[i] = permIn[idx[i]]
+// (idx[i] without permIn) records where the other records of slot i still
+// are.  With x set every record moves; x, v, v_in and id are then read at
+// permIn[idx[i]] when permIn is set (their builds were deferred).
 struct Permute {
     const uint32_t* idx;
     const pbf_float4 *x, *p, *v, *vin;
     const uint32_t* id;
     pbf_float4 *xo, *po, *vo, *vino;
     uint32_t* ido;
+    const uint32_t* permIn = nullptr;
+    uint32_t* permOut = nullptr;
 };
 // launch_cell_ranges (tileEnd required) and the permutation in one pass.
 pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
