@@ -102,6 +102,8 @@ FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 2
 TESTING = {"build_lists"}  # passes that run the radius test
 TRAFFIC_SUMMARY = REPO / "profiles" / "build_lists_dam_break_1M_traffic.md"
 TRAFFIC_KERNEL = "build_lists"
+COUNTER_SUMMARY = REPO / "profiles" / "list_passes_dam_break_1M_counters.md"
+CUS, CLOCK_HZ = 256, 2.4e9  # MI355X compute units and peak engine clock
 
 
 def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
@@ -128,6 +130,16 @@ def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) 
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
             + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
             + PHASE_BYTES["confine_resolve"])
+
+
+def measured_valu_insts() -> float | None:
+    """SQ_INSTS_VALU per launch of the list build from the committed
+    counter summary."""
+    if not COUNTER_SUMMARY.exists():
+        return None
+    m = re.search(r"valu_insts_per_launch k_build_lists:\s*([0-9.eE+]+)",
+                  COUNTER_SUMMARY.read_text())
+    return float(m.group(1)) if m else None
 
 
 def measured_traffic() -> float | None:
@@ -421,6 +433,13 @@ def main() -> int:
             valu = {"kernel": dominant, "achieved": tf, "peak": VALU_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": tf / VALU_PEAK_TFLOPS,
                     "alg_flops_per_launch": flops}
+            insts = measured_valu_insts() if dominant == TRAFFIC_KERNEL else None
+            if insts is not None:
+                # The list build's bound: VALU issue, one wave-instruction
+                # per CU per cycle over every CU at the peak clock.
+                issue_ms = insts / (CUS * CLOCK_HZ) * 1e3
+                valu["issue"] = {"insts_per_launch": insts, "issue_ms": issue_ms,
+                                 "frac": issue_ms / drow["avg_ms"]}
 
     bstep = step_bytes_per_particle(scene.params.solverIterations, key_bits, cells, n)
     step_roof = {"bytes_per_particle_step": bstep,
