@@ -191,39 +191,51 @@ private:
     hipEvent_t done_[2] = {nullptr, nullptr};
 };
 
-// True for memory the device can reach directly (pinned host or device).
-bool device_visible(const void* p) {
+// Where a caller's host-array argument lives: pageable host memory (staged
+// through the pinned chunks), memory both the host and the device reach
+// (pinned or managed: one DMA), or device memory, which the host-array
+// calls reject (the host cannot read or write it).
+enum class HostArray { Pageable, Pinned, Device };
+
+HostArray host_array_kind(const void* p) {
     hipPointerAttribute_t a{};
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError();  // pageable memory reports an error
-        return false;
+        return HostArray::Pageable;
     }
-    return a.type == hipMemoryTypeHost || a.type == hipMemoryTypeDevice ||
-           a.type == hipMemoryTypeManaged;
+    if (a.type == hipMemoryTypeHost || a.type == hipMemoryTypeManaged) return HostArray::Pinned;
+    if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeArray) return HostArray::Device;
+    return HostArray::Pageable;
 }
 
 }  // namespace
 
 // What the adjoint needs of one substep of the forward, in the working-set
-// order of each grid build (the four constraint iterations, then the
-// post-solve rebuild): the particle id of every slot, the grouped lists
-// and counts, the positions entering and leaving each iteration (lambda in
-// .w of the former), and the post-solve phase's records.
+// order of each grid build (one per constraint iteration, then the
+// post-solve rebuild at index `iterations`): the particle id of every slot,
+// the grouped lists and counts, the positions entering and leaving each
+// iteration (lambda in .w of the former), and the post-solve phase's
+// records.  Sized for the solver's iteration count.
 struct SubstepTape {
-    DeviceBuffer<std::uint32_t> ids[5];
-    DeviceBuffer<std::uint32_t> nbr[5];
-    DeviceBuffer<int> cnt[5];
-    DeviceBuffer<float4> pk[4], pout[4];
+    std::vector<DeviceBuffer<std::uint32_t>> ids, nbr;
+    std::vector<DeviceBuffer<int>> cnt;
+    std::vector<DeviceBuffer<float4>> pk, pout;
     DeviceBuffer<float4> xs, pf, v3, omega, vin;
 
-    void allocate(std::size_t n, std::size_t listWords) {
-        if (xs.size() >= n && nbr[0].size() >= listWords) return;
-        for (int b = 0; b < 5; ++b) {
+    void allocate(std::size_t n, std::size_t listWords, int iterations) {
+        const std::size_t builds = static_cast<std::size_t>(iterations) + 1;
+        if (xs.size() >= n && nbr.size() == builds && nbr[0].size() >= listWords) return;
+        ids.resize(builds);
+        nbr.resize(builds);
+        cnt.resize(builds);
+        pk.resize(builds - 1);
+        pout.resize(builds - 1);
+        for (std::size_t b = 0; b < builds; ++b) {
             ids[b].allocate(n);
             nbr[b].allocate(listWords);
             cnt[b].allocate(n);
         }
-        for (int k = 0; k < 4; ++k) {
+        for (std::size_t k = 0; k + 1 < builds; ++k) {
             pk[k].allocate(n);
             pout[k].allocate(n);
         }
@@ -336,6 +348,8 @@ struct PBFSolver::State {
     void tape_build(int b) {
         SubstepTape* tp = tape();
         if (tp == nullptr) return;
+        if (b < 0 || static_cast<std::size_t>(b) >= tp->ids.size())
+            throw std::logic_error("Adjoint tape holds fewer builds than the substep runs");
         tape_copy(tp->ids[b], id, count);
         launch_check(pbfint::tape_lists(nbr.data(), tp->nbr[b].data(), cnt.data(), count, stride,
                                         kMaxNeighbors / 4, stream));
@@ -414,6 +428,7 @@ struct PBFSolver::State {
             (void)hipStreamEndCapture(stream, &graph);
             if (graph != nullptr) (void)hipGraphDestroy(graph);
             permute(perm_between(buffer_key(), key));
+            recordsLag = false;  // nothing ran: every record is where the frame found it
             throw;
         }
         const hipError_t ce = hipStreamEndCapture(stream, &graph);
@@ -537,6 +552,7 @@ struct PBFSolver::State {
         const float h = P.smoothingRadius;
         const pbfdev::Colliders col = collision.state().view();
         const float half = 0.45f * skin * h;
+        recordsLag = false;  // skin builds move every record
         pbfint::ListPass sk = lpass();
         sk.stale2 = half * half;
         sk.exactMax = kMaxNeighbors;
@@ -673,7 +689,7 @@ struct PBFSolver::State {
             if (post) {
                 build_and_reorder();
                 build_lists();
-                tape_build(4);
+                tape_build(P.solverIterations);
                 if (SubstepTape* tp = tape()) {
                     tape_copy(tp->xs, x, n);
                     tape_copy(tp->pf, p, n);
@@ -730,7 +746,11 @@ struct PBFSolver::State {
     // chunk's DMA).  Throws, with the stream drained, at the first
     // non-finite coordinate.
     void upload_checked(const float4* src, float4* dst, std::size_t n) {
-        if (device_visible(src)) {
+        const HostArray kind = host_array_kind(src);
+        if (kind == HostArray::Device)
+            throw std::invalid_argument(
+                "Particle arrays must be host memory (use setParticlesDevice for device arrays)");
+        if (kind == HostArray::Pinned) {
             for (std::size_t k = 0; k < n; ++k)
                 if (!finite4(src[k]))
                     throw std::invalid_argument(
@@ -766,7 +786,11 @@ struct PBFSolver::State {
     // else through the pinned stage, the host copy of one chunk overlapping
     // the DMA of the next.  Returns with the copy complete.
     void download(const float4* src, float4* out, std::size_t n) {
-        if (device_visible(out)) {
+        const HostArray kind = host_array_kind(out);
+        if (kind == HostArray::Device)
+            throw std::invalid_argument(
+                "Destination must be host memory (use the copy-to-device calls for device arrays)");
+        if (kind == HostArray::Pinned) {
             hip_check(hipMemcpyAsync(out, src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
             return;
         }
@@ -846,6 +870,12 @@ struct PBFSolver::State {
             if (out == nullptr) {
                 hip_check(hipStreamSynchronize(stream));
                 throw std::invalid_argument("Destination cannot be null");
+            }
+            if (host_array_kind(out) == HostArray::Device) {
+                hip_check(hipStreamSynchronize(stream));
+                throw std::invalid_argument(
+                    "Destination must be host memory (use the copy-to-device calls for device "
+                    "arrays)");
             }
             download(scratch.data(), out, n);
         } else {
@@ -1513,7 +1543,8 @@ void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
                 const std::size_t subs =
                     (frameCount - lastFirst) * static_cast<std::size_t>(s.params.substeps);
                 if (m.tapes.size() < subs) m.tapes.resize(subs);
-                for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(m.n, s.nbr.size());
+                for (std::size_t q = 0; q < subs; ++q)
+                    m.tapes[q].allocate(m.n, s.nbr.size(), s.params.solverIterations);
                 s.tapes = &m.tapes;
                 s.tapeNext = 0;
             }
@@ -1587,7 +1618,8 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
         if (k != m.tapedSegment) {
             // Replay the segment with the tape on.
             if (m.tapes.size() < subs) m.tapes.resize(subs);
-            for (std::size_t q = 0; q < subs; ++q) m.tapes[q].allocate(n, s.nbr.size());
+            for (std::size_t q = 0; q < subs; ++q)
+                m.tapes[q].allocate(n, s.nbr.size(), P.solverIterations);
             m.tapedSegment = static_cast<std::size_t>(-1);
             m.restore(m.checkpoints[k]);
             s.reset_status();
@@ -1609,7 +1641,7 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
             pbfint::AdjBuffers w{m.pb.data(),   m.vb.data(),    m.x0b.data(),  m.v1b.data(),
                                  m.v3b.data(),  m.etab.data(),  m.wb.data(),   m.prb.data(),
                                  m.gsum.data(), m.cterm.data(), m.lamb.data(), m.acc.data()};
-            const pbfint::AdjPost post{build(tp, 4), tp.xs.data(), tp.pf.data(), tp.v3.data(),
+            const pbfint::AdjPost post{build(tp, P.solverIterations), tp.xs.data(), tp.pf.data(), tp.v3.data(),
                                        tp.omega.data(), tp.vin.data()};
             launch_check(pbfint::adjoint_post(post, w, n, ap, s.stream));
             for (int it = P.solverIterations; it-- > 0;) {

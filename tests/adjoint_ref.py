@@ -5,7 +5,7 @@ Test infrastructure, not product code.  The forward follows the step the
 library runs (src/pbf/pbf_solver.cu:186-328 and the kernels it launches):
 
     v1 = v + g dt;  p = x + v1 dt                          integrate
-    4 x:  lambda_i = -C_i / (S_i + eps_l)                   computeLambda
+    I x:  lambda_i = -C_i / (S_i + eps_l)                   computeLambda
           p_i += (m / rho0) sum_j (l_i + l_j + s_ij) G(p_i - p_j), then the
           container projection                              Delta-p, collisions
     v2 = (p - x) / dt                                       velocity update
@@ -44,6 +44,7 @@ class Params:
     box_max: tuple
     restitution: float = 0.0
     friction: float = 0.0
+    iterations: int = 4
 
 
 # ---------------------------------------------------------------------------
@@ -179,8 +180,8 @@ def delta_p(p, lam, I, J, P):
 
 
 def forward(x, v, pairs, P, c, e, keep_aux=False, frozen=None):
-    """One substep.  pairs: five (I, J) arrays, four iterations then the
-    post-solve build.  `frozen` (the aux of a base evaluation) holds the
+    """One substep.  pairs: P.iterations + 1 (I, J) arrays, one per
+    constraint iteration, then the post-solve build.  `frozen` (the aux of a base evaluation) holds the
     discrete decisions fixed as well: which coordinates the container
     projects, which walls a particle touches and which contacts act, and
     whether |eta| passes the normalisation threshold -- the function the
@@ -189,7 +190,8 @@ def forward(x, v, pairs, P, c, e, keep_aux=False, frozen=None):
     v1 = v + g * P.dt
     p = x + v1 * P.dt
     aux = {"x": x, "v1": v1, "it": []}
-    for k in range(4):
+    iters = len(pairs) - 1
+    for k in range(iters):
         I, J = pairs[k]
         L = lambda_terms(p, I, J, P)
         dp = delta_p(p, L["lam"], I, J, P)
@@ -200,7 +202,7 @@ def forward(x, v, pairs, P, c, e, keep_aux=False, frozen=None):
             pn = np.where(keep > 0, p + dp, fk["out"])
         aux["it"].append(dict(p=p, lam=L["lam"], keep=keep, out=pn))
         p = pn
-    I, J = pairs[4]
+    I, J = pairs[iters]
     n = len(p)
     d = p[I] - p[J]
     W = poly6(d, P)
@@ -245,7 +247,8 @@ def backward(aux, pairs, P, c, e, xb, vb):
     velocities), the adjoints of its inputs x, v and of c, e."""
     x, v1, p = aux["x"], aux["v1"], aux["p"]
     n = len(x)
-    I, J = pairs[4]
+    iters = len(aux["it"])
+    I, J = pairs[iters]
     d = p[I] - p[J]
     W = poly6(d, P)
     G = spiky(d, P)
@@ -302,7 +305,7 @@ def backward(aux, pairs, P, c, e, xb, vb):
     pb += scatter(n, I, db) - scatter(n, J, db)
     # constraint iterations in reverse
     gs = P.mass / P.rho0
-    for k in reversed(range(4)):
+    for k in reversed(range(iters)):
         it = aux["it"][k]
         pk, lam, keep = it["p"], it["lam"], it["keep"]
         I, J = pairs[k]
@@ -357,7 +360,7 @@ def substep_full(x, v, P, c, e):
     g = np.array(P.gravity)
     p = x + (v + g * P.dt) * P.dt
     pairs = []
-    for _ in range(4):
+    for _ in range(P.iterations):
         pairs.append(pairs_within(p, P.h))
         L = lambda_terms(p, *pairs[-1], P)
         p, _ = container(p + delta_p(p, L["lam"], *pairs[-1], P), P)

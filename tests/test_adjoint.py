@@ -23,12 +23,12 @@ def _pairs(p, h):
     return I, J
 
 
-def _setup(seed, n=140, restitution=0.0, friction=0.0):
+def _setup(seed, n=140, restitution=0.0, friction=0.0, iterations=4):
     rng = np.random.default_rng(seed)
     P = A.Params(dt=1.0 / 120.0, h=0.1, mass=0.789532, rho0=6378.0, eps_lambda=141.2,
                  gravity=(0.0, -9.8, 0.0), scorr_k=1e-3, scorr_n=4, scorr_dq=0.02,
                  radius=0.025, box_min=(0.0, 0.0, 0.0), box_max=(0.5, 1.0, 0.3),
-                 restitution=restitution, friction=friction)
+                 restitution=restitution, friction=friction, iterations=iterations)
     # a jittered lattice at s = h / 2 against the floor and two walls, so
     # that collisions and contacts act
     k = np.arange(n)
@@ -39,22 +39,25 @@ def _setup(seed, n=140, restitution=0.0, friction=0.0):
 
 
 def _frozen_pairs(x, v, P, c, e):
-    """The five builds' pairs of the base forward, from its own positions."""
-    pairs = [None] * 5
+    """The builds' pairs of the base forward (one per iteration, then the
+    post-solve build), from its own positions."""
+    pairs = [None] * (P.iterations + 1)
     # iteration k's pairs come from the positions entering it
     g = np.array(P.gravity)
     p = x + (v + g * P.dt) * P.dt
-    for k in range(4):
+    for k in range(P.iterations):
         pairs[k] = _pairs(p, P.h)
         L = A.lambda_terms(p, *pairs[k], P)
         p, _ = A.container(p + A.delta_p(p, L["lam"], *pairs[k], P), P)
-    pairs[4] = _pairs(p, P.h)
+    pairs[P.iterations] = _pairs(p, P.h)
     return pairs
 
 
-@pytest.mark.parametrize("seed,restitution,friction", [(1, 0.0, 0.0), (2, 0.3, 0.2), (3, 0.0, 0.5)])
-def test_substep_adjoint_matches_central_differences(seed, restitution, friction):
-    P, x, v = _setup(seed, restitution=restitution, friction=friction)
+@pytest.mark.parametrize("seed,restitution,friction,iterations",
+                         [(1, 0.0, 0.0, 4), (2, 0.3, 0.2, 4), (3, 0.0, 0.5, 4), (4, 0.2, 0.1, 5),
+                          (5, 0.3, 0.2, 2), (6, 0.0, 0.0, 6)])
+def test_substep_adjoint_matches_central_differences(seed, restitution, friction, iterations):
+    P, x, v = _setup(seed, restitution=restitution, friction=friction, iterations=iterations)
     c, e = 0.01, 1e-3
     pairs = _frozen_pairs(x, v, P, c, e)
     rng = np.random.default_rng(seed + 100)

@@ -19,11 +19,12 @@ from pbf_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _case(nx, substeps, seed=5):
+def _case(nx, substeps, seed=5, iterations=4):
     sc = scenes.small_block(nx, nx, nx, substeps=substeps)
     P = sc.params
     P.xsphViscosity = 1e-4
     P.vorticityStrength = 1e-4
+    P.solverIterations = iterations
     rng = np.random.default_rng(seed)
     v0 = sc.velocities.copy()
     v0[:, :3] = rng.normal(0.0, 0.1, (sc.n, 3)).astype(np.float32)
@@ -31,15 +32,16 @@ def _case(nx, substeps, seed=5):
                  rho0=P.restDensity, eps_lambda=P.lambdaEpsilon,
                  gravity=(P.gravity.x, P.gravity.y, P.gravity.z), scorr_k=P.scorrK,
                  scorr_n=P.scorrN, scorr_dq=P.scorrDeltaQ, radius=P.particleRadius,
-                 box_min=sc.minBounds, box_max=sc.maxBounds)
+                 box_min=sc.minBounds, box_max=sc.maxBounds, iterations=iterations)
     return sc, P, Q, v0, rng
 
 
-@pytest.mark.parametrize("nx,substeps,frames,every", [(10, 1, 4, 2), (12, 2, 5, 2)])
-def test_adjoint_matches_fp64_central_differences(gpu, nx, substeps, frames, every):
+@pytest.mark.parametrize("nx,substeps,frames,every,iterations",
+                         [(10, 1, 4, 2, 4), (12, 2, 5, 2, 4), (10, 1, 3, 2, 5), (10, 1, 3, 3, 6)])
+def test_adjoint_matches_fp64_central_differences(gpu, nx, substeps, frames, every, iterations):
     import pbf_amd
 
-    sc, P, Q, v0, rng = _case(nx, substeps)
+    sc, P, Q, v0, rng = _case(nx, substeps, iterations=iterations)
     x0 = sc.positions
     target = x0[:, :3].astype(np.float64) + np.array([0.2, 0.0, 0.0])
 
@@ -78,8 +80,8 @@ def test_adjoint_matches_fp64_central_differences(gpu, nx, substeps, frames, eve
             ad.append(float(v0b[k, d]))
     fd, ad = np.array(fd), np.array(ad)
     err = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
-    print(f"[{sc.n} particles, {S} substeps] v0: {len(fd)} coordinates, relative error {err:.2e}",
-          flush=True)
+    print(f"[{sc.n} particles, {S} substeps, {iterations} iterations] v0: {len(fd)} coordinates, "
+          f"relative error {err:.2e}", flush=True)
     assert err <= 1e-3
     # The parameters move every particle, so a difference step of 1e-3 or
     # 1e-2 of their value already carries some contact decisions across

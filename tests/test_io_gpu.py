@@ -70,3 +70,29 @@ def test_pinned_upload_and_copy_out(gpu):
     with pytest.raises(pbf_amd.InvalidArgument):
         s.setParticles(bad, v)
     assert np.array_equal(s.copyPositionsToHost(), x), "kept after a rejected pinned upload"
+
+
+def test_host_array_calls_reject_device_memory(gpu):
+    """setParticles and the host copy-outs read or write their arrays on the
+    host, so a device pointer is an invalid argument (the device-array calls
+    take those), and the state is left as it was."""
+    import pbf_amd
+
+    n = 1000
+    s = pbf_amd.PBFSolver()
+    s.initialize(n, MIN, MAX, _params())
+    x = _records(5, n, 0.5, 7.5, 1.0)
+    v = _records(6, n, -1.0, 1.0, 0.0)
+    s.setParticles(x, v)
+    d = pbf_amd.DeviceArray.from_host(x)
+    L = pbf_amd.lib()
+    assert L.pbf_solver_set_particles(s._h, d.ptr, v.ctypes.data, n) == 1
+    assert L.pbf_solver_set_particles(s._h, x.ctypes.data, d.ptr, n) == 1
+    assert L.pbf_solver_copy_positions_to_host(s._h, d.ptr, n) == 1
+    assert L.pbf_solver_copy_velocities_to_host(s._h, d.ptr, n) == 1
+    assert np.array_equal(s.copyPositionsToHost(), x)
+    assert np.array_equal(s.copyVelocitiesToHost(), v)
+    s.setParticlesDevice(d, pbf_amd.DeviceArray.from_host(v), n)
+    out = pbf_amd.DeviceArray(16 * n)
+    s.copyVelocitiesToDevice(out, n)
+    assert np.array_equal(out.to_numpy(np.float32, (n, 4)), v)
