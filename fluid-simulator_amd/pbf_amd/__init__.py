@@ -211,6 +211,7 @@ _SIGNATURES = {
     "pbf_grid_destroy": (_st, [_vp]),
     "pbf_grid_initialize": (_st, [_vp, _sz, float3, float3, _f]),
     "pbf_grid_build": (_st, [_vp, _vp, _sz]),
+    "pbf_grid_build_static": (_st, [_vp, _sz, float3, float3, _f, _vp, _vp, _vp, _vp, _sz, _vp]),
     "pbf_grid_sorted_keys": (_vp, [_vp]),
     "pbf_grid_sorted_indices": (_vp, [_vp]),
     "pbf_grid_cell_start": (_vp, [_vp]),

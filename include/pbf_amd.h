@@ -275,6 +275,26 @@ pbf_status pbf_grid_initialize(pbf_grid* grid, size_t maxParticles, pbf_float3 m
 /* SpatialGrid::build (spatial_grid.cu:139-226): keys, stable sort, ranges.
  * Synchronises before returning, like the reference. */
 pbf_status pbf_grid_build(pbf_grid* grid, const pbf_float4* d_predicted, size_t particleCount);
+/* Entry point that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
+ * pbf_grid_build without a grid handle: the grid is given by minBounds,
+ * maxBounds and cellSize (validated like pbf_grid_initialize), the outputs
+ * are the caller's (d_sortedKeys and d_sortedIndices hold particleCount
+ * entries, d_cellStart and d_cellEnd cellCapacity >= the grid's cells), and
+ * the keys, the sort's ping-pong buffers and its counters live in one
+ * static device array of about 13.6 MB per GPU, held until the process
+ * ends.  Results equal pbf_grid_build's bit for bit.  At most
+ * PBF_GRID_BUILD_STATIC_MAX_PARTICLES particles (PBF_ERR_LENGTH above).
+ * Asynchronous on `stream` (no host synchronisation, no memset node).
+ * Calls on one GPU share its array: calls that overlap on one GPU (other
+ * streams or threads) race; calls ordered on one stream do not. */
+#define PBF_GRID_BUILD_STATIC_MAX_PARTICLES 1048576u
+pbf_status pbf_grid_build_static(const pbf_float4* d_predicted, size_t particleCount,
+                                 pbf_float3 minBounds, pbf_float3 maxBounds, float cellSize,
+                                 uint32_t* d_sortedKeys, uint32_t* d_sortedIndices,
+                                 int* d_cellStart, int* d_cellEnd, size_t cellCapacity,
+                                 void* stream);
 /* Device-pointer getters (spatial_grid.cu:230-244); valid until the next
  * initialize/build. */
 const uint32_t* pbf_grid_sorted_keys(const pbf_grid* grid);
