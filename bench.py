@@ -45,7 +45,18 @@ Prints one JSON line:
                  forward): the adjoint's passes against a plain run of the
                  same frames (--no-adjoint skips it)
   cpu_baseline   the FP32 OpenMP restatement of the reference dataflow on
-                 the host cores, on a bounded sample of the same workload
+                 the host cores (built on the machine that runs it with
+                 -march=native, OMP_PROC_BIND=close, the CPU model and thread
+                 count recorded), on a bounded sample of the same workload;
+                 `samples` adds the 32K dam break (every step of its 100) and
+                 the 8M double dam break (substeps after warm-up)
+
+`value` is the median over --reps runs of exactly --steps timed frames, each
+run started from the same warmed-up state (the reference benchmark takes the
+median of 5 runs, benchmark/pbf_benchmark.cu:158-211); `runs_ms_per_step`
+lists every run.  `ms_per_substep` = `ms_per_step` / substeps per frame.
+`headline_8m_step_roofline` repeats the 8M double dam break's compulsory-byte
+fraction at the top level: the step fraction the 8M target is quoted on.
 
 `--copy-calibration BYTES` instead copies BYTES of device memory once with
 the library's device-to-device copy: the known-size copy whose HBM counters
@@ -55,6 +66,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import re
 import statistics
 import sys
@@ -106,7 +118,11 @@ COUNTER_SUMMARY = REPO / "profiles" / "list_passes_dam_break_1M_counters.md"
 CUS, CLOCK_HZ = 256, 2.4e9  # MI355X compute units and peak engine clock
 
 
-def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
+def alg_bytes(kernel: str, n: int, key_bits: int, cells: int,
+              kmean: float | None = None) -> float | None:
+    """Algorithmic bytes of one launch.  With kmean (the mean neighbour
+    count measured over the timed frames), the list term 4 K of the list
+    passes uses it instead of the count at rest."""
     if kernel == "radix_sort":
         return 16.0 * ((key_bits + 7) // 8) * n
     if kernel == "cell_ranges":
@@ -114,7 +130,11 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int) -> float | None:
     if kernel == "fill_cells":
         return 4.0 * cells
     b = PHASE_BYTES.get(kernel)
-    return None if b is None else float(b) * n
+    if b is None:
+        return None
+    if kmean is not None and kernel in FLOPS_PER_NEIGHBOUR:
+        b += 4.0 * (kmean - KMEAN)
+    return float(b) * n
 
 
 def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) -> float:
@@ -151,34 +171,106 @@ def measured_traffic() -> float | None:
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(scene, seconds_budget: float = 25.0) -> dict:
-    sys.path.insert(0, str(REPO / "oracle"))
-    from oracle import OracleSolver, threads
+NATIVE_FLAGS = ["-O3", "-march=native", "-fPIC", "-shared", "-std=c11", "-fopenmp",
+                "-ffp-contract=off", "-fno-fast-math", "-DORC_REAL=float"]
 
-    s = OracleSolver("f32")
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _native_oracle() -> tuple:
+    """The FP32 OpenMP oracle compiled for this host with -march=native (in a
+    temporary directory), or the prebuilt x86-64-v3 library when gcc fails."""
+    import subprocess
+    import tempfile
+
+    out = Path(tempfile.mkdtemp(prefix="pbf_cpu_")) / "liboracle_f32_native.so"
+    src = REPO / "oracle" / "pbf_oracle.c"
+    r = subprocess.run(["gcc", *NATIVE_FLAGS, str(src), "-o", str(out), "-lm"],
+                       capture_output=True, text=True)
+    if r.returncode == 0:
+        return str(out), "gcc " + " ".join(NATIVE_FLAGS)
+    return None, "prebuilt liboracle_f32.so (-march=x86-64-v3): native build failed"
+
+
+def _time_oracle(OracleSolver, scene, warmup: int, timed: int) -> tuple:
+    s = OracleSolver("f32_native")
     s.initialize(scene.n, scene.minBounds, scene.maxBounds, scene.params)
     s.setParticles(scene.positions, scene.velocities)
-    t = time.perf_counter()
-    s.step()  # warm-up
-    first = time.perf_counter() - t
-    reps = max(1, min(5, int(seconds_budget / max(first, 1e-3)) - 1))
+    for _ in range(warmup):
+        s.step()
     times = []
-    for _ in range(reps):
+    for _ in range(timed):
         t = time.perf_counter()
         s.step()
         times.append(time.perf_counter() - t)
-    med = statistics.median(times)
-    sub = scene.params.substeps
-    return {
-        "value": scene.n * sub / med,
-        "unit": "particle-steps/s",
-        "cores": threads(),
-        "kind": "port",
-        "sample": f"{scene.name}: {scene.n} particles, 1 warm-up frame then median of {reps} "
-                  f"timed frames ({sub} substeps each) of the FP32 OpenMP restatement "
-                  f"(5 grid builds per substep, slot-major neighbour lists, one loop per "
-                  f"reference kernel)",
-    }
+    return times
+
+
+def cpu_samples() -> dict:
+    """Child-process entry (--cpu-samples): the FP32 OpenMP restatement of
+    the reference dataflow (5 grid builds per substep, slot-major neighbour
+    lists, one loop per reference kernel) on the three dam breaks."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle
+    from oracle import OracleSolver, threads
+    from pbf_amd import scenes
+
+    path, build = _native_oracle()
+    oracle.load("f32_native", path or str(REPO / "oracle" / "liboracle_f32.so"))
+    cores = threads()
+    common = {"unit": "particle-steps/s", "cores": cores, "kind": "port",
+              "cpu_model": cpu_model(), "build": build,
+              "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+              "omp_places": os.environ.get("OMP_PLACES")}
+    out = {}
+    # 1M (the measured workload): one warm-up frame, median of 5 frames.
+    sc = scenes.dam_break(2)
+    t = _time_oracle(OracleSolver, sc, 1, 5)
+    out["dam_break_2"] = dict(common, value=sc.n * sc.params.substeps / statistics.median(t),
+                              sample=f"{sc.name}: {sc.n} particles, 1 warm-up frame then the "
+                                     f"median of 5 timed frames of {sc.params.substeps} "
+                                     "substeps", seconds=sum(t))
+    # 32K: every step of the workload's 100.
+    sc = scenes.dam_break(1)
+    t = _time_oracle(OracleSolver, sc, 0, 100)
+    out["dam_break_1"] = dict(common, value=sc.n * sc.params.substeps * 100 / sum(t),
+                              sample=f"{sc.name}: {sc.n} particles, all 100 steps timed",
+                              seconds=sum(t))
+    # 8M: substeps of the same per-substep work (one substep per frame),
+    # two warm-up substeps then the median of 5.
+    sc = scenes.dam_break(3, substeps=1)
+    t = _time_oracle(OracleSolver, sc, 2, 5)
+    out["dam_break_3"] = dict(common, value=sc.n / statistics.median(t),
+                              sample=f"{sc.name}: {sc.n} particles at one substep per frame "
+                                     "(the per-substep work of its 3-substep frames), 2 "
+                                     "warm-up substeps then the median of 5 timed",
+                              seconds=sum(t))
+    return out
+
+
+def cpu_baseline() -> dict:
+    """Runs cpu_samples in a child process with OMP_PROC_BIND=close and
+    OMP_PLACES=cores set before its OpenMP runtime starts (and its memory,
+    about 10 GB for the 8M lists, returned when it exits)."""
+    import subprocess
+
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-samples"],
+                       capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        return {"error": r.stderr[-2000:]}
+    samples = json.loads(r.stdout.strip().splitlines()[-1])
+    head = dict(samples["dam_break_2"])
+    head["samples"] = {k: v for k, v in samples.items() if k != "dam_break_2"}
+    return head
 
 
 def copy_calibration(nbytes: int) -> int:
@@ -201,15 +293,21 @@ def copy_calibration(nbytes: int) -> int:
 
 
 def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=None,
-               tune_frames: int = 0):
+               tune_frames: int = 0, reps: int = 1, count_every: int = 0):
     """Warm-up frames, then (tune_frames > 0) the library's autotuner from
     the warmed-up state (it restores that state), or the given tuning, then
-    `steps` timed frames.  Returns (seconds, kernel times, tuning report)."""
+    `reps` runs of `steps` timed frames, each started from the warmed-up
+    state (copied out after the warm-up and set again before the run, so
+    every run computes the same frames).  With count_every > 0 the last run
+    also reads the neighbour counts every count_every frames (outside its
+    timing: that run is then not timed).  Returns (seconds per run, kernel
+    times, tuning report, mean neighbour count or None)."""
     solver = pbf_amd.PBFSolver()
     solver.initialize(scene.n, scene.minBounds, scene.maxBounds, scene.params)
     solver.setParticles(scene.positions, scene.velocities)
     solver.run(warmup)
     pbf_amd.synchronize()
+    x0, v0 = solver.copyPositionsToHost(), solver.copyVelocitiesToHost()
     report = None
     if tune_frames > 0:
         chosen, table = solver.autotune(tune_frames)
@@ -220,16 +318,29 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=No
     elif tuning is not None:
         solver.setTuning(tuning["neighbor_skin"], tuning["pass_block"])
     stale0 = solver.staleFrames()
-    if profile:
-        solver.setProfiling(True)
-    t0 = time.perf_counter()
-    solver.run(steps)
-    pbf_amd.synchronize()
-    elapsed = time.perf_counter() - t0
+    runs = []
+    for _ in range(reps):
+        solver.setParticles(x0, v0)
+        if profile:
+            solver.setProfiling(True)
+        pbf_amd.synchronize()
+        t0 = time.perf_counter()
+        solver.run(steps)
+        pbf_amd.synchronize()
+        runs.append(time.perf_counter() - t0)
     times = solver.kernelTimes() if profile else {}
     if report is not None:
         report["stale_frames_timed"] = solver.staleFrames() - stale0
-    return elapsed, times, report
+    kmean = None
+    if count_every > 0:
+        solver.setProfiling(False)
+        solver.setParticles(x0, v0)
+        counts = []
+        for f in range(0, steps, count_every):
+            solver.run(min(count_every, steps - f))
+            counts.append(float(solver.neighborLists(0)[1].mean()))
+        kmean = statistics.fmean(counts)
+    return runs, times, report, kmean
 
 
 def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2) -> dict:
@@ -277,13 +388,14 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
             "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
 
 
-def double_dam_break(pbf_amd, scenes, tuning, steps: int = 10, warmup: int = 3) -> dict:
+def double_dam_break(pbf_amd, scenes, tuning, steps: int = 100, warmup: int = 3) -> dict:
     """The 8,388,608-particle double dam break (two 128 x 256 x 128 blocks
     at opposite walls, 3 substeps per frame) with the tuning chosen on the
-    1M workload: particle-steps/s over `steps` frames after `warmup`, and
-    the compulsory-byte roofline of that rate."""
+    1M workload: particle-steps/s over the workload's 100 frames after
+    `warmup`, and the compulsory-byte roofline of that rate."""
     sc = scenes.dam_break(3)
-    elapsed, _, _ = run_frames(pbf_amd, sc, warmup, steps, profile=False, tuning=tuning)
+    runs, _, _, _ = run_frames(pbf_amd, sc, warmup, steps, profile=False, tuning=tuning)
+    elapsed = runs[0]
     sub = sc.params.substeps
     value = sc.n * steps * sub / elapsed
     grid = pbf_amd.SpatialGrid()
@@ -294,6 +406,7 @@ def double_dam_break(pbf_amd, scenes, tuning, steps: int = 10, warmup: int = 3) 
     return {"workload": f"{sc.name}: {sc.n} particles, 4 iterations, {sub} substeps per "
                         f"1/60 s frame, frames {warmup + 1}-{warmup + steps}",
             "value": value, "unit": "particle-steps/s", "ms_per_step": elapsed * 1e3 / steps,
+            "ms_per_substep": elapsed * 1e3 / steps / sub,
             "b_alg_roofline": {"bytes_per_particle_step": b_alg,
                                "achieved": b_alg * value / 1e9, "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": b_alg * value / 1e9 / HBM_PEAK_GBS},
@@ -372,7 +485,14 @@ def main() -> int:
                     help="time the default tuning (the reference's builds, 256-thread passes)")
     ap.add_argument("--tune-frames", type=int, default=3,
                     help="frames the autotuner times per candidate")
+    ap.add_argument("--reps", type=int, default=5,
+                    help="timed runs of --steps frames; value is their median")
+    ap.add_argument("--cpu-samples", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
+
+    if a.cpu_samples:  # child process of cpu_baseline(); touches no GPU
+        print(json.dumps(cpu_samples()), flush=True)
+        return 0
 
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -392,8 +512,10 @@ def main() -> int:
     # One timed "step" is one PBFSolver::step() call, i.e. one frame of
     # `substeps` PBF substeps; throughput counts particle-substeps (the
     # reference benchmark's particle-substeps/s, pbf_benchmark.cu:201-211).
-    elapsed, _, tuning = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=False,
-                                    tune_frames=0 if a.no_autotune else a.tune_frames)
+    runs, _, tuning, _ = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=False,
+                                    tune_frames=0 if a.no_autotune else a.tune_frames,
+                                    reps=max(1, a.reps))
+    elapsed = statistics.median(runs)
     value = n * a.steps * substeps / elapsed
 
     grid = pbf_amd.SpatialGrid()
@@ -403,12 +525,14 @@ def main() -> int:
 
     kernel_rows = {}
     roofline = valu = None
+    kmean = None
     if not a.no_profile_pass:
-        _, times, _ = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=True,
-                                 tuning=tuning["chosen"] if tuning else None)
+        _, times, _, kmean = run_frames(pbf_amd, scene, a.warmup, a.steps, profile=True,
+                                        tuning=tuning["chosen"] if tuning else None,
+                                        count_every=10)
         for name, (ms, launches) in times.items():
             avg = ms / max(launches, 1)
-            b = alg_bytes(name, n, key_bits, cells)
+            b = alg_bytes(name, n, key_bits, cells, kmean)
             kernel_rows[name] = {"ms_total": ms, "launches": launches, "avg_ms": avg,
                                  "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
         dominant = max(kernel_rows, key=lambda k: kernel_rows[k]["ms_total"])
@@ -423,11 +547,12 @@ def main() -> int:
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": measured_traffic() if dominant == TRAFFIC_KERNEL else None,
             "avg_launch_ms": drow["avg_ms"],
-            "alg_bytes_per_launch": alg_bytes(dominant, n, key_bits, cells),
+            "alg_bytes_per_launch": alg_bytes(dominant, n, key_bits, cells, kmean),
             "launches": drow["launches"],
+            "mean_neighbours": kmean,
         }
         if dominant in FLOPS_PER_NEIGHBOUR:
-            flops = n * (KMEAN * FLOPS_PER_NEIGHBOUR[dominant]
+            flops = n * ((kmean or KMEAN) * FLOPS_PER_NEIGHBOUR[dominant]
                          + (CAND * 8 if dominant in TESTING else 0))
             tf = flops / (drow["avg_ms"] * 1e-3) / 1e12
             valu = {"kernel": dominant, "achieved": tf, "peak": VALU_PEAK_TFLOPS,
@@ -454,7 +579,10 @@ def main() -> int:
 
     headline_8m = None
     if a.config == 2 and not a.no_8m:
-        headline_8m = double_dam_break(pbf_amd, scenes, tuning["chosen"] if tuning else None)
+        try:
+            headline_8m = double_dam_break(pbf_amd, scenes, tuning["chosen"] if tuning else None)
+        except pbf_amd.PbfError as e:  # reported in the line rather than losing it
+            headline_8m = {"error": f"{type(e).__name__}: {e}"}
 
     batched = None
     if not a.no_batched:
@@ -464,7 +592,7 @@ def main() -> int:
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(scene)
+        cpu = cpu_baseline()
 
     out = {
         "metric": "particle-steps/s @1M (4 iters) + % HBM roofline",
@@ -477,6 +605,8 @@ def main() -> int:
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": elapsed * 1e3 / a.steps,
+        "ms_per_substep": elapsed * 1e3 / a.steps / substeps,
+        "runs_ms_per_step": [r * 1e3 / a.steps for r in runs],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -492,6 +622,10 @@ def main() -> int:
         "valu": valu,
         "step_roofline": step_roof,
         "b_alg_roofline": b_alg_roof,
+        "headline_8m_step_roofline": (dict(headline_8m["b_alg_roofline"],
+                                           workload=headline_8m["workload"],
+                                           value=headline_8m["value"])
+                                      if headline_8m and "value" in headline_8m else None),
         "double_dam_break_8m": headline_8m,
         "batched": batched,
         "adjoint": adjoint,

@@ -197,11 +197,15 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
-def load(precision: str = "f64") -> C.CDLL:
+def load(precision: str = "f64", path: str | None = None) -> C.CDLL:
+    """The oracle library of `precision` ("f64", "f32"); with `path`, the
+    library at that path (a host-native build of the FP32 form, say) is
+    loaded under that name instead."""
     if precision not in _LIBS:
-        path = HERE / f"liboracle_{precision}.so"
-        if not path.exists():
-            build()
+        if path is None:
+            path = HERE / f"liboracle_{precision}.so"
+            if not path.exists():
+                build()
         lib = C.CDLL(str(path))
         for name, (res, args) in _SIGS.items():
             fn = getattr(lib, name)

@@ -159,3 +159,85 @@ def test_taped_forward_equals_plain_run(gpu):
     (xa, va), (xb, vb) = runs
     assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
     assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+def _differentiable():
+    import pbf_amd
+
+    sc, target = scenes.differentiable_block()
+    s = pbf_amd.PBFSolver()
+    s.initialize(sc.n, sc.minBounds, sc.maxBounds, sc.params)
+    s.setParticles(sc.positions, sc.velocities)
+    return sc, target, s
+
+
+def _loss_bar(x, target, n):
+    xbar = np.zeros_like(x)
+    xbar[:, :3] = (2.0 * (x[:, :3] - target) / n).astype(np.float32)
+    return xbar
+
+
+def test_differentiable_workload_matches_fp64_differences(gpu):
+    """The 65,536-particle differentiable workload over 5 frames with
+    checkpoints every 2 (backward replays two segments from their
+    checkpoints): the gradients with respect to 32 seeded initial-velocity
+    coordinates, the XSPH viscosity and the vorticity strength against FP64
+    central differences of the same rollout (tests/golden/adjoint_fd_64k_f5.npz,
+    written by tests/golden/make_adjoint_fd.py)."""
+    from pathlib import Path
+
+    import pbf_amd
+
+    fx = np.load(Path(__file__).resolve().parent / "golden" / "adjoint_fd_64k_f5.npz")
+    frames = int(fx["frames"])
+    sc, target, s = _differentiable()
+    adj = pbf_amd.PBFAdjoint(s)
+    adj.forward(frames, 2)
+    x = s.copyPositionsToHost()
+    loss = float(np.mean(np.sum((x[:, :3] - target) ** 2, 1)))
+    _, v0b, cb, eb = adj.backward(_loss_bar(x, target, sc.n), None)
+    ad = v0b[fx["particles"], fx["axes"]].astype(np.float64)
+    err = np.linalg.norm(ad - fx["fd_v"]) / np.linalg.norm(fx["fd_v"])
+    worst = np.argsort(-np.abs(ad - fx["fd_v"]))[:4]
+    print("  worst coordinates (adjoint, fd, fd at the finer step): " + "; ".join(
+        f"{ad[k]:+.5e} {fx['fd_v'][k]:+.5e} {fx['fd_v_fine'][k]:+.5e}" for k in worst),
+        flush=True)
+    print(f"[65536 particles, {frames} frames] loss {loss:.9e} (FP64 {float(fx['loss']):.9e}); "
+          f"32 velocity coordinates: relative error {err:.2e}; d/dc {cb:+.6e} "
+          f"(fd {float(fx['fd_c']):+.6e}); d/de {eb:+.6e} (fd {float(fx['fd_e']):+.6e})",
+          flush=True)
+    assert abs(loss - float(fx["loss"])) <= 1e-5 * float(fx["loss"])
+    assert err <= 1e-3
+    assert abs(cb - float(fx["fd_c"])) <= 1e-3 * abs(float(fx["fd_c"]))
+    assert abs(eb - float(fx["fd_e"])) <= 1e-3 * abs(float(fx["fd_e"]))
+
+
+def test_differentiable_workload_60_frames_properties(gpu):
+    """The full 60-frame rollout with checkpoints every 10 (five segments
+    replayed).  Its gradient leaves FP32's range (it grows by orders of
+    magnitude per frame past 10 frames), so the checks are properties: the
+    taped forward ends where a plain run ends, bit for bit; a zero loss
+    gradient gives zero gradients; backward is bitwise repeatable and
+    leaves the solver where forward ended."""
+    import pbf_amd
+
+    sc, target, plain = _differentiable()
+    plain.run(60)
+    xp, vp = plain.copyPositionsToHost(), plain.copyVelocitiesToHost()
+    del plain
+    _, _, s = _differentiable()
+    adj = pbf_amd.PBFAdjoint(s)
+    adj.forward(60, 10)
+    x, v = s.copyPositionsToHost(), s.copyVelocitiesToHost()
+    assert np.array_equal(x.view(np.uint32), xp.view(np.uint32))
+    assert np.array_equal(v.view(np.uint32), vp.view(np.uint32))
+    x0b, v0b, cb, eb = adj.backward(np.zeros_like(x), None)
+    assert not np.any(x0b) and not np.any(v0b) and cb == 0.0 and eb == 0.0
+    xbar = _loss_bar(x, target, sc.n)
+    g1 = adj.backward(xbar, None)
+    g2 = adj.backward(xbar, None)
+    assert np.array_equal(g1[0].view(np.uint32), g2[0].view(np.uint32))
+    assert np.array_equal(g1[1].view(np.uint32), g2[1].view(np.uint32))
+    assert np.array_equal(s.copyPositionsToHost().view(np.uint32), x.view(np.uint32))
+    print(f"65536 particles, 60 frames: |dL/dv0| {np.linalg.norm(g1[1][:, :3]):.3e}, "
+          f"dL/dc {g1[2]:.3e}", flush=True)

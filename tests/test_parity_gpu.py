@@ -83,12 +83,13 @@ def test_grid_bitexact_random_with_clamping(gpu, oracle64):
     assert np.array_equal(cen, oce)
 
 
-@pytest.mark.parametrize("cells_axis", [(6, 5, 4), (40, 30, 20), (160, 100, 64), (400, 300, 200)])
+@pytest.mark.parametrize("cells_axis", [(6, 5, 4), (16, 16, 8), (40, 30, 20), (160, 100, 64),
+                                       (400, 300, 200)])
 def test_grid_bitexact_every_pass_count(gpu, oracle64, cells_axis):
-    """One to four radix passes (7, 15, 20 and 25 key bits), 300K keys in
-    293 tiles, ten builds in a row on one grid: the sort's tile tickets,
-    histograms and look-back words are reused from build to build, and no
-    pass reads the array it writes."""
+    """One to four radix passes (7, 11, 15, 20 and 25 key bits; 11 bits end
+    in a 3-bit pass), 300K keys in 293 tiles, ten builds in a row on one
+    grid: the sort's histograms and scan words are reused from build to
+    build, and no pass reads the array it writes."""
     rng = np.random.default_rng(sum(cells_axis))
     n = 300_001
     mx = tuple(0.1 * c for c in cells_axis)
@@ -221,7 +222,7 @@ def _contact_mask(p, sc, band=1e-3):
     return m
 
 
-def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True):
+def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True, norm_bar=1e-5):
     """Strict bar away from contacts; at contacts 1e-3 relative, at most 1%
     of the particles outside the strict bar; the norm bar over all.  With
     contacts_only=False a branch flip may also reach the flipped particle's
@@ -245,7 +246,7 @@ def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True):
         assert np.all(strict | contact), "a particle away from every contact misses the bar"
     assert np.all(loose), "a particle differs by more than 1e-3"
     assert outside.sum() <= max(1, len(pg) // 100)
-    assert norm_rel <= 1e-5
+    assert norm_rel <= norm_bar
     assert np.array_equal(pg[:, 3], po[:, 3]) and np.array_equal(vg[:, 3], vo[:, 3])
 
 
