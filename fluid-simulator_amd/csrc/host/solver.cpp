@@ -213,13 +213,14 @@ HostArray host_array_kind(const void* p) {
 // What the adjoint needs of one substep of the forward, in the working-set
 // order of each grid build (one per constraint iteration, then the
 // post-solve rebuild at index `iterations`): the particle id of every slot,
-// the grouped lists and counts, the positions entering and leaving each
-// iteration (lambda in .w of the former), and the post-solve phase's
-// records.  Sized for the solver's iteration count.
+// the grouped lists and counts, the positions entering each iteration
+// (lambda in .w) and after its Delta-p, before the collision solve (written
+// by the Delta-p pass itself), and the post-solve phase's records.  Sized
+// for the solver's iteration count.
 struct SubstepTape {
     std::vector<DeviceBuffer<std::uint32_t>> ids, nbr;
     std::vector<DeviceBuffer<int>> cnt;
-    std::vector<DeviceBuffer<float4>> pk, pout;
+    std::vector<DeviceBuffer<float4>> pk, pre;
     DeviceBuffer<float4> xs, pf, v3, omega, vin;
 
     void allocate(std::size_t n, std::size_t listWords, int iterations) {
@@ -229,7 +230,7 @@ struct SubstepTape {
         nbr.resize(builds);
         cnt.resize(builds);
         pk.resize(builds - 1);
-        pout.resize(builds - 1);
+        pre.resize(builds - 1);
         for (std::size_t b = 0; b < builds; ++b) {
             ids[b].allocate(n);
             nbr[b].allocate(listWords);
@@ -237,7 +238,7 @@ struct SubstepTape {
         }
         for (std::size_t k = 0; k + 1 < builds; ++k) {
             pk[k].allocate(n);
-            pout[k].allocate(n);
+            pre[k].allocate(n);
         }
         for (DeviceBuffer<float4>* b : {&xs, &pf, &v3, &omega, &vin}) b->allocate(n);
     }
@@ -642,7 +643,13 @@ struct PBFSolver::State {
         const float invDt = 1.0f / sdt;
         const float h = P.smoothingRadius;
         const pbfdev::Colliders col = collision.state().view();
-        const bool post = P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f;
+        // The reference rebuilds after the solve only for XSPH or vorticity
+        // (pbf_solver.cu:260).  Under the adjoint every frame rebuilds, taped
+        // or not, so that a replayed segment follows the frames forward()
+        // ran, and the derivatives with respect to c and epsilon at 0 have
+        // the final positions' lists.
+        const bool post =
+            P.xsphViscosity != 0.0f || P.vorticityStrength != 0.0f || adjointExact;
         recordsLag = false;
         for (int substep = 0; substep < P.substeps; ++substep) {
             {
@@ -678,12 +685,12 @@ struct PBFSolver::State {
                     dl.clearCells = grid.cellStart.data();
                     dl.clearCount = grid.numCells;
                     dl.xperm = recordsLag ? perm.data() : nullptr;
+                    if (SubstepTape* tp = tape()) dl.prePos = tp->pre[it].data();
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
                         stride, col, P.particleRadius, scenes(), dl, status.data(), stream));
                 }
-                if (SubstepTape* tp = tape()) tape_copy(tp->pout[it], pa, n);
                 std::swap(p, pa);
             }
             if (post) {
@@ -729,6 +736,15 @@ struct PBFSolver::State {
                 }
                 std::swap(v, va);
             } else {
+                if (SubstepTape* tp = tape()) {
+                    // Confinement is off, but its strength's derivative at 0
+                    // needs the vorticity: computed for the tape only (the
+                    // pass changes no state; p takes its records).
+                    launch_check(pbfint::launch_vorticity(
+                        x.data(), v.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h,
+                        omega.data(), p.data(), stride, lpass(), status.data(), stream));
+                    tape_copy(tp->omega, omega, n);
+                }
                 Scope sc(prof, "resolve_velocities", stream);
                 launch_check(pbfint::launch_resolve_guarded(
                     x.data(), vin.data(), v.data(), n, col, P.particleRadius,
@@ -1513,11 +1529,6 @@ void PBFAdjoint::forward(std::size_t frameCount, std::size_t checkpointEvery) {
     if (s.maxParticles == 0) throw std::logic_error("PBFSolver must be initialized before running");
     if (s.sceneCount != 0) throw std::invalid_argument("The adjoint takes a single-scene solver");
     if (checkpointEvery == 0) throw std::invalid_argument("checkpointEvery must be positive");
-    const pbfdev::Colliders col = s.collision.state().view();
-    if (col.sphereCount != 0 || col.boxCount != 0 || col.planeCount != 0)
-        throw std::invalid_argument("The adjoint models the container only");
-    if (!(s.params.vorticityStrength > 0.0f))
-        throw std::invalid_argument("The adjoint needs vorticity confinement enabled");
     m.frames = frameCount;
     m.every = checkpointEvery;
     m.n = s.count;
@@ -1604,8 +1615,7 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
     ap.scorrK = P.scorrK;
     ap.scorrN = P.scorrN;
     ap.scorrDq = P.scorrDeltaQ;
-    const Container& box = s.collision.state().hostContainer;
-    ap.box = pbf_container{box.min, box.max};
+    ap.col = s.collision.state().view();
     const auto build = [&](SubstepTape& tp, int b) {
         return pbfint::AdjBuild{tp.nbr[b].data(), tp.cnt[b].data(), tp.ids[b].data(), s.stride};
     };
@@ -1645,7 +1655,7 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
                                        tp.omega.data(), tp.vin.data()};
             launch_check(pbfint::adjoint_post(post, w, n, ap, s.stream));
             for (int it = P.solverIterations; it-- > 0;) {
-                const pbfint::AdjIter iter{build(tp, it), tp.pk[it].data(), tp.pout[it].data()};
+                const pbfint::AdjIter iter{build(tp, it), tp.pk[it].data(), tp.pre[it].data()};
                 launch_check(pbfint::adjoint_iteration(iter, w, n, ap, s.stream));
             }
             launch_check(pbfint::adjoint_predict(w, n, ap.dt, s.stream));

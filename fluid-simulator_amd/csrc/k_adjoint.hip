@@ -1,9 +1,10 @@
 // Adjoint of the PBF substep for gfx950: reverse-mode derivatives of one
-// substep (integrate, 4 x [lambda, Delta-p with s_corr, container
-// projection], velocity update, XSPH, vorticity, confinement, contact
-// velocities) with the neighbour lists and every discrete decision of the
-// forward held fixed (which coordinates the container projects, which walls
-// a particle touches and which contacts act).
+// substep (integrate, the constraint iterations [lambda, Delta-p with
+// s_corr, the collision solve against the container, spheres, rounded boxes
+// and planes], velocity update, XSPH, vorticity, confinement, contact
+// velocities against every collider) with the neighbour lists and every
+// discrete decision of the forward held fixed (which projections act and
+// their branches, which contacts touch and act).
 //
 // The forward's tape gives, per grid build, the slot -> particle-id map, the
 // grouped lists and counts, and the positions of that build's working-set
@@ -17,8 +18,8 @@
 // atomic; only the two parameter sums do (per wave, in FP64).
 //
 // Reference of the differentiated forward: src/pbf/pbf_solver.cu:186-328,
-// src/pbf/pbf_solver_kernels.cu:26-385, src/pbf/collision_kernels.cu:21-82,
-// :236-282, include/pbf/sph_helpers.cuh:13-95.
+// src/pbf/pbf_solver_kernels.cu:26-385, src/pbf/collision_kernels.cu:7-371,
+// src/pbf/collision_system.cu:133-218, include/pbf/sph_helpers.cuh:13-95.
 #include "pbf_device.hpp"
 #include "pbf_internal.hpp"
 #include "pbf_launch.hpp"
@@ -92,12 +93,167 @@ struct Nbr2 {
     float4 a, b;
 };
 
-struct Walls {
-    float minX, maxX, minY, minZ, maxZ;
+// A 3x3 matrix by rows.
+struct M3 {
+    float3 r0, r1, r2;
 };
 
-__device__ __forceinline__ Walls walls_of(const pbf_container& k, float r) {
-    return Walls{k.min.x + r, k.max.x - r, k.min.y + r, k.min.z + r, k.max.z - r};
+__device__ __forceinline__ M3 m3_identity() {
+    return M3{make_float3(1.0f, 0.0f, 0.0f), make_float3(0.0f, 1.0f, 0.0f),
+              make_float3(0.0f, 0.0f, 1.0f)};
+}
+
+// A B.
+__device__ __forceinline__ M3 m3_mul(const M3& A, const M3& B) {
+    const auto row = [&](float3 a) {
+        return make_float3(a.x * B.r0.x + a.y * B.r1.x + a.z * B.r2.x,
+                           a.x * B.r0.y + a.y * B.r1.y + a.z * B.r2.y,
+                           a.x * B.r0.z + a.y * B.r1.z + a.z * B.r2.z);
+    };
+    return M3{row(A.r0), row(A.r1), row(A.r2)};
+}
+
+// A^T b.
+__device__ __forceinline__ float3 m3_tmul(const M3& A, float3 b) {
+    return b.x * A.r0 + b.y * A.r1 + b.z * A.r2;
+}
+
+// scale (I - u u^T).
+__device__ __forceinline__ M3 radial_jac(float3 u, float scale) {
+    return M3{scale * (make_float3(1.0f, 0.0f, 0.0f) - u.x * u),
+              scale * (make_float3(0.0f, 1.0f, 0.0f) - u.y * u),
+              scale * (make_float3(0.0f, 0.0f, 1.0f) - u.z * u)};
+}
+
+// Zeroes the rows of J whose coordinate differs between a and b (a
+// coordinate a projection set to a constant).
+__device__ __forceinline__ void zero_changed_rows(M3& J, const float4& a, const float4& b) {
+    if (a.x != b.x) J.r0 = make_float3(0.0f, 0.0f, 0.0f);
+    if (a.y != b.y) J.r1 = make_float3(0.0f, 0.0f, 0.0f);
+    if (a.z != b.z) J.r2 = make_float3(0.0f, 0.0f, 0.0f);
+}
+
+// The collision fixed point of CollisionSystem::solve (collision_fixed_point)
+// replayed from q with the Jacobian of every projection that acts
+// accumulated: J = J_m ... J_1 for the m projections that move q, each with
+// its decisions (which projection acts, the sphere's centre case, the box's
+// clamped axes or interior face) held fixed.
+//   container: the clamped coordinates' rows vanish;
+//   sphere, q' = c + (md / |o|) o: (md / |o|) (I - u u^T); at o = 0 the x
+//     row vanishes;
+//   box outside its core, q' = cp + (r / |o|) o, cp = clamp(q):
+//     diag(inside) + (r / |o|) (I - u u^T) diag(outside); inside the core the
+//     face coordinate's row vanishes;
+//   plane, q' = q + n (r - n.(q - p0)): I - n n^T.
+__device__ __forceinline__ M3 collide_jacobian(float4 q, const Colliders& c, float r) {
+    M3 J = m3_identity();
+    for (int pass = 0; pass < kMaxCollisionPasses; ++pass) {
+        bool marked = false;
+        float4 before = q;
+        if (project_container(q, *c.container, r)) {
+            zero_changed_rows(J, before, q);
+            marked = true;
+        }
+        for (int i = 0; i < c.sphereCount; ++i) {
+            before = q;
+            const pbf_sphere sp = c.spheres[i];
+            if (!project_sphere(q, sp, r)) continue;
+            marked = true;
+            const float3 o = make_float3(before.x - sp.center.x, before.y - sp.center.y,
+                                         before.z - sp.center.z);
+            const float d2 = dot3(o, o);
+            if (d2 == 0.0f) {
+                J.r0 = make_float3(0.0f, 0.0f, 0.0f);
+            } else {
+                const float inv = 1.0f / __builtin_sqrtf(d2);
+                J = m3_mul(radial_jac(inv * o, (sp.radius + r) * inv), J);
+            }
+        }
+        for (int i = 0; i < c.boxCount; ++i) {
+            before = q;
+            const pbf_box b = c.boxes[i];
+            if (!project_box(q, b, r)) continue;
+            marked = true;
+            const float3 mn = make_float3(b.center.x - b.halfExtents.x,
+                                          b.center.y - b.halfExtents.y,
+                                          b.center.z - b.halfExtents.z);
+            const float3 mx = make_float3(b.center.x + b.halfExtents.x,
+                                          b.center.y + b.halfExtents.y,
+                                          b.center.z + b.halfExtents.z);
+            const float3 cp = make_float3(fminf(fmaxf(before.x, mn.x), mx.x),
+                                          fminf(fmaxf(before.y, mn.y), mx.y),
+                                          fminf(fmaxf(before.z, mn.z), mx.z));
+            const float3 o = f3(before) - cp;
+            const float d2 = dot3(o, o);
+            if (d2 == 0.0f) {
+                zero_changed_rows(J, before, q);
+            } else {
+                const float inv = 1.0f / __builtin_sqrtf(d2);
+                M3 Jk = radial_jac(inv * o, r * inv);
+                // columns of the clamped (outside) axes only, plus the
+                // identity on the axes the clamp passes through
+                const bool inX = o.x == 0.0f, inY = o.y == 0.0f, inZ = o.z == 0.0f;
+                const auto mask = [&](float3 row, float3 unit) {
+                    return make_float3(inX ? unit.x : row.x, inY ? unit.y : row.y,
+                                       inZ ? unit.z : row.z);
+                };
+                Jk.r0 = mask(Jk.r0, make_float3(1.0f, 0.0f, 0.0f));
+                Jk.r1 = mask(Jk.r1, make_float3(0.0f, 1.0f, 0.0f));
+                Jk.r2 = mask(Jk.r2, make_float3(0.0f, 0.0f, 1.0f));
+                J = m3_mul(Jk, J);
+            }
+        }
+        for (int i = 0; i < c.planeCount; ++i) {
+            const pbf_plane pl = c.planes[i];
+            if (!project_plane(q, pl, r)) continue;
+            marked = true;
+            const float l2 = dot3(make_float3(pl.normal.x, pl.normal.y, pl.normal.z),
+                                  make_float3(pl.normal.x, pl.normal.y, pl.normal.z));
+            const float ns = 1.0f / __builtin_sqrtf(l2);
+            const float3 nu = make_float3(pl.normal.x * ns, pl.normal.y * ns, pl.normal.z * ns);
+            J = m3_mul(radial_jac(nu, 1.0f), J);
+        }
+        if (!marked) break;
+    }
+    return J;
+}
+
+// One acting contact of resolveVelocitiesKernel as the reverse pass needs
+// it: its geometry, target normal speed, whether the incoming velocity
+// points into the surface, and the velocity entering it.
+struct ActingContact {
+    ContactGeom g;
+    float target;
+    bool neg;
+    float3 v;
+};
+
+// Replays the contacts on v (resolve_velocity's arithmetic, contact by
+// contact) and counts those that act.  With stop >= 0 it returns as soon as
+// acting contact number `stop` is reached, without applying it, and
+// describes it in *out.  Returns the number of acting contacts applied.
+__device__ __forceinline__ int replay_contacts(const float4& p, const float4& vin, float4& v,
+                                               const Colliders& c, float r, float restitution,
+                                               float friction, int stop, ActingContact* out) {
+#pragma clang fp contract(off)
+    const float ts = 1.0f - friction;
+    int count = 0;
+    bool done = false;
+    for_each_contact(p, c, r, [&](const ContactGeom& g) {
+        if (done) return;
+        const float vn = v.x * g.nx + v.y * g.ny + v.z * g.nz;
+        const float vinn = vin.x * g.nx + vin.y * g.ny + vin.z * g.nz;
+        const float target = vinn < 0.0f ? -restitution * vinn : 0.0f;
+        if (vn > target || (vn == target && vinn >= 0.0f)) return;
+        if (count == stop) {
+            *out = ActingContact{g, target, vinn < 0.0f, f3(v)};
+            done = true;
+            return;
+        }
+        contact_velocity(g.nx, g.ny, g.nz, vin, restitution, ts, v);
+        ++count;
+    });
+    return count;
 }
 
 // ---------------------------------------------------------------------------
@@ -112,15 +268,22 @@ struct PostArgs {
     const pbf_float4* vin;  // velocities after gravity (contact inputs)
     size_t n;
     float h, dt, visc, vort, radius, restitution, friction;
-    pbf_container box;
+    Colliders col;
 };
 
 // Confinement and contact velocities: recomputes eta, N and v4, replays the
-// wall sequence of resolveVelocities, and reverses both.
+// contact sequence of resolveVelocities (walls, spheres, boxes, planes),
+// and reverses both.  A contact maps v to t n + ts (v - (v.n) n) with
+// t = -e (v_in.n) when v_in.n < 0; the reverse of acting contact m needs
+// the velocity entering it, which a replay of the first m contacts gives
+// (contacts are few per particle).  Sphere and box normals follow the
+// position, n = o / |o|, so those contacts also add to the position's
+// adjoint: (I - n n^T) n-bar / |o| on the axes o follows.
 __global__ void __launch_bounds__(kBlock)
     k_adj_post1(PostArgs a, const pbf_float4* __restrict__ vb, pbf_float4* __restrict__ v1b,
                 pbf_float4* __restrict__ v3b, pbf_float4* __restrict__ etab,
-                pbf_float4* __restrict__ wb, double* __restrict__ acc) {
+                pbf_float4* __restrict__ wb, pbf_float4* __restrict__ pb,
+                double* __restrict__ acc) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     float epart = 0.0f;
     if (i < a.n) {
@@ -139,39 +302,38 @@ __global__ void __launch_bounds__(kBlock)
         const float3 N = ok ? (1.0f / el) * eta : make_float3(0.0f, 0.0f, 0.0f);
         const float3 v3i = f3(ld4(a.v3, i));
         const float3 v4 = v3i + (a.dt * a.vort) * cross3(N, w);
-        // Replay the walls of resolveVelocities on v4.
-        const Walls wl = walls_of(a.box, a.radius);
-        constexpr float eps = 1e-4f;
-        const float3 vi1 = f3(ld4(a.vin, i));
-        const float3 nrm[5] = {make_float3(1, 0, 0), make_float3(-1, 0, 0), make_float3(0, 1, 0),
-                               make_float3(0, 0, 1), make_float3(0, 0, -1)};
-        bool touch[5];
-        touch[0] = pi.x <= wl.minX + eps;
-        touch[1] = !touch[0] && pi.x >= wl.maxX - eps;
-        touch[2] = pi.y <= wl.minY + eps;
-        touch[3] = pi.z <= wl.minZ + eps;
-        touch[4] = !touch[3] && pi.z >= wl.maxZ - eps;
+        const float4 pf4 = ld4(a.pf, i);
+        const float4 vin4 = ld4(a.vin, i);
+        const float3 vi1 = f3(vin4);
+        const float4 v4w = make_float4(v4.x, v4.y, v4.z, 0.0f);
+        float4 vtmp = v4w;
+        const int acting = replay_contacts(pf4, vin4, vtmp, a.col, a.radius, a.restitution,
+                                           a.friction, -1, nullptr);
         const float ts = 1.0f - a.friction;
-        bool act[5], inNeg[5];
-        float3 v = v4;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const float vn = dot3(v, nrm[k]);
-            const float vinn = dot3(vi1, nrm[k]);
-            const float target = vinn < 0.0f ? -a.restitution * vinn : 0.0f;
-            inNeg[k] = vinn < 0.0f;
-            act[k] = touch[k] && !(vn > target || (vn == target && vinn >= 0.0f));
-            if (act[k]) v = target * nrm[k] + ts * (v - vn * nrm[k]);
-        }
         float3 vc = f3(ld4(vb, gid));
         float3 b1 = make_float3(0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int k = 4; k >= 0; --k) {
-            if (!act[k]) continue;
-            const float vnb = dot3(vc, nrm[k]);
-            if (inNeg[k]) b1 = b1 + (-a.restitution * vnb) * nrm[k];
-            vc = ts * (vc - vnb * nrm[k]);
+        float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
+        for (int m = acting; m-- > 0;) {
+            ActingContact ac;
+            vtmp = v4w;
+            replay_contacts(pf4, vin4, vtmp, a.col, a.radius, a.restitution, a.friction, m, &ac);
+            const float3 nrm = make_float3(ac.g.nx, ac.g.ny, ac.g.nz);
+            const float vnb = dot3(vc, nrm);  // adjoint of the target speed
+            if (ac.neg) b1 = b1 + (-a.restitution * vnb) * nrm;
+            if (ac.g.radial) {
+                const float3 nb = (ac.neg ? -a.restitution * vnb : 0.0f) * vi1 + ac.target * vc -
+                                  (ts * vnb) * ac.v - (ts * dot3(ac.v, nrm)) * vc;
+                const float3 g = ac.g.inv_len * (nb - dot3(nrm, nb) * nrm);
+                pacc = pacc + make_float3(ac.g.outX ? g.x : 0.0f, ac.g.outY ? g.y : 0.0f,
+                                          ac.g.outZ ? g.z : 0.0f);
+            }
+            vc = ts * (vc - vnb * nrm);
         }
+        float4 pbi = ld4(pb, gid);
+        pbi.x += pacc.x;
+        pbi.y += pacc.y;
+        pbi.z += pacc.z;
+        st4(pb, gid, pbi);
         st4(v1b, gid, make_float4(b1.x, b1.y, b1.z, 0.0f));
         // confinement: v4 = v3 + dt e (N x w)
         st4(v3b, i, make_float4(vc.x, vc.y, vc.z, 0.0f));
@@ -299,29 +461,25 @@ __global__ void __launch_bounds__(kBlock)
 struct IterArgs {
     pbfint::AdjBuild b;
     const pbf_float4* pk;    // positions entering the iteration, lambda in .w
-    const pbf_float4* pout;  // positions leaving it (after the projection)
+    const pbf_float4* pre;   // positions after Delta-p, before the collision solve
     size_t n;
     float h, mass, rho0, eps, sk, sdq, radius;
     int sn;
-    pbf_container box;
+    Colliders col;
 };
 
-// The projection's mask, then the lambda terms (density and gradient sums
-// in FP64, as the forward) for the next two passes.
+// The collision solve's reverse (J^T of the replayed fixed point from the
+// taped pre-projection position), then the lambda terms (density and
+// gradient sums in FP64, as the forward) for the next two passes.
 __global__ void __launch_bounds__(kBlock)
     k_adj_it1(IterArgs a, pbf_float4* __restrict__ pb, pbf_float4* __restrict__ prb,
               pbf_float4* __restrict__ gsum, float* __restrict__ cterm) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t gid = a.b.ids[i];
-    const Walls wl = walls_of(a.box, a.radius);
-    const float4 po = ld4(a.pout, i);
-    // A coordinate the container moved sits exactly on its wall.
-    const bool kx = !(po.x == wl.minX || po.x == wl.maxX);
-    const bool ky = !(po.y == wl.minY);
-    const bool kz = !(po.z == wl.minZ || po.z == wl.maxZ);
-    const float4 p = ld4(pb, gid);
-    const float4 r = make_float4(kx ? p.x : 0.0f, ky ? p.y : 0.0f, kz ? p.z : 0.0f, 0.0f);
+    const M3 J = collide_jacobian(ld4(a.pre, i), a.col, a.radius);
+    const float3 rr = m3_tmul(J, f3(ld4(pb, gid)));
+    const float4 r = make_float4(rr.x, rr.y, rr.z, 0.0f);
     st4(pb, gid, r);
     st4(prb, i, r);
     const Sph s = sph_consts(a.h);
@@ -485,12 +643,12 @@ pbf_status adjoint_post(const AdjPost& t, const AdjBuffers& w, size_t n, const A
     a.radius = prm.radius;
     a.restitution = prm.restitution;
     a.friction = prm.friction;
-    a.box = prm.box;
+    a.col = prm.col;
     const unsigned g = grid_for(n);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     pbf_status rc;
     hipLaunchKernelGGL(k_adj_post1, dim3(g), dim3(kBlock), 0, st, a, w.vb, w.v1b, w.v3b, w.etab,
-                       w.wb, w.acc);
+                       w.wb, w.pb, w.acc);
     if ((rc = pbf_hip_check(hipGetLastError())) != PBF_OK) return rc;
     hipLaunchKernelGGL(k_adj_post2, dim3(g), dim3(kBlock), 0, st, a, w.etab, w.wb, w.pb);
     if ((rc = pbf_hip_check(hipGetLastError())) != PBF_OK) return rc;
@@ -505,7 +663,7 @@ pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n, co
     IterArgs a;
     a.b = t.build;
     a.pk = t.pk;
-    a.pout = t.pout;
+    a.pre = t.pre;
     a.n = n;
     a.h = prm.h;
     a.mass = prm.mass;
@@ -515,7 +673,7 @@ pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n, co
     a.sn = prm.scorrN;
     a.sdq = prm.scorrDq;
     a.radius = prm.radius;
-    a.box = prm.box;
+    a.col = prm.col;
     const unsigned g = grid_for(n);
     pbf_status st;
     hipLaunchKernelGGL(k_adj_it1, dim3(g), dim3(kBlock), 0, static_cast<hipStream_t>(stream), a,

@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(kBlock)
                           const pbf_float4* p0, float stale2, uint32_t* __restrict__ key,
                           pbf_float3 kmn, float kcs, pbf_int3 kgs, int* __restrict__ clearCells,
                           size_t clearCount, const uint32_t* __restrict__ xperm,
-                          uint32_t* __restrict__ status) {
+                          pbf_float4* __restrict__ prePos, uint32_t* __restrict__ status) {
     clear_cells(clearCells, clearCount);
     const size_t i = thread_index<true>();
     if (i >= n) return;
@@ -604,6 +604,7 @@ __global__ void __launch_bounds__(kBlock)
     pi.y += dpy;
     pi.z += dpz;
     pi.w = x[xperm != nullptr ? xperm[i] : i].w;
+    if (prePos != nullptr) st4(prePos, i, pi);
     const Colliders ci = colliders_of<kBatched>(col, i, st);
     if (!collision_fixed_point(pi, ci, radius))
         atomicOr(status, PBF_STATUS_COLLISION_NONCONVERGENCE);
@@ -961,7 +962,7 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
     PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin,  \
                pout, nbr, cnt, maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, \
                st, lp.p0, lp.stale2, lp.keyOut, lp.keyMin, lp.keyCell, lp.keyDims,          \
-               lp.clearCells, lp.clearCount, lp.xperm, status)
+               lp.clearCells, lp.clearCount, lp.xperm, lp.prePos, status)
     if (batched && skinned) PBF_DELTA(true, true);
     if (batched) PBF_DELTA(true, false);
     if (skinned) PBF_DELTA(false, true);

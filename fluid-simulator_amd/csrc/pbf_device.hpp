@@ -358,12 +358,25 @@ __device__ __forceinline__ void contact_velocity(float nx, float ny, float nz, c
     v.z = target * nz + tangentScale * tz;
 }
 
-// resolveVelocitiesKernel body (collision_kernels.cu:236-371).
-__device__ __forceinline__ void resolve_velocity(const float4& p, const float4& vin, float4& v,
-                                                 const Colliders& c, float r, float restitution,
-                                                 float friction) {
+// The geometry of one contact of resolveVelocitiesKernel: its normal, and
+// for normals that follow the position (spheres, and rounded boxes outside
+// their core) the reciprocal distance 1 / |o| and the axes along which o
+// follows the position (o = p - clamp(p) for boxes).
+struct ContactGeom {
+    float nx, ny, nz;
+    bool radial;
+    float inv_len;
+    bool outX, outY, outZ;
+};
+
+// Calls visit(geom) for every contact of resolveVelocitiesKernel
+// (collision_kernels.cu:236-371) that p touches, in the reference's order:
+// the container walls (floor, -x or +x, -z or +z; the top is open), then
+// every sphere, box and plane in array order.
+template <class Visit>
+__device__ __forceinline__ void for_each_contact(const float4& p, const Colliders& c, float r,
+                                                 Visit visit) {
 #pragma clang fp contract(off)
-    const float ts = 1.0f - friction;
     constexpr float eps = 1e-4f;  // collisionEpsilon (collision_kernels.cu:254)
     const pbf_container& k = *c.container;
     const float minX = k.min.x + r;
@@ -371,15 +384,18 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
     const float minY = k.min.y + r;
     const float minZ = k.min.z + r;
     const float maxZ = k.max.z - r;
+    const auto wall = [&](float nx, float ny, float nz) {
+        visit(ContactGeom{nx, ny, nz, false, 0.0f, false, false, false});
+    };
     if (p.x <= minX + eps)
-        contact_velocity(1.0f, 0.0f, 0.0f, vin, restitution, ts, v);
+        wall(1.0f, 0.0f, 0.0f);
     else if (p.x >= maxX - eps)
-        contact_velocity(-1.0f, 0.0f, 0.0f, vin, restitution, ts, v);
-    if (p.y <= minY + eps) contact_velocity(0.0f, 1.0f, 0.0f, vin, restitution, ts, v);
+        wall(-1.0f, 0.0f, 0.0f);
+    if (p.y <= minY + eps) wall(0.0f, 1.0f, 0.0f);
     if (p.z <= minZ + eps)
-        contact_velocity(0.0f, 0.0f, 1.0f, vin, restitution, ts, v);
+        wall(0.0f, 0.0f, 1.0f);
     else if (p.z >= maxZ - eps)
-        contact_velocity(0.0f, 0.0f, -1.0f, vin, restitution, ts, v);
+        wall(0.0f, 0.0f, -1.0f);
 
     for (int i = 0; i < c.sphereCount; ++i) {
         const pbf_sphere s = c.spheres[i];
@@ -391,7 +407,7 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
         const float cd = md + eps;
         if (d2 > 0.0f && d2 <= cd * cd) {
             const float sc = 1.0f / __builtin_sqrtf(d2);
-            contact_velocity(ox * sc, oy * sc, oz * sc, vin, restitution, ts, v);
+            visit(ContactGeom{ox * sc, oy * sc, oz * sc, true, sc, true, true, true});
         }
     }
     for (int i = 0; i < c.boxCount; ++i) {
@@ -409,28 +425,24 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
         const float oy = p.y - cy;
         const float oz = p.z - cz;
         const float d2 = ox * ox + oy * oy + oz * oz;
-        float nx, ny, nz;
         const float rc = r + eps;
         if (d2 > 0.0f && d2 <= rc * rc) {
             const float sc = 1.0f / __builtin_sqrtf(d2);
-            nx = ox * sc;
-            ny = oy * sc;
-            nz = oz * sc;
+            visit(ContactGeom{ox * sc, oy * sc, oz * sc, true, sc, ox != 0.0f, oy != 0.0f,
+                              oz != 0.0f});
         } else if (d2 == 0.0f) {
             const float dx = fminf(p.x - mnx, mxx - p.x);
             const float dy = fminf(p.y - mny, mxy - p.y);
             const float dz = fminf(p.z - mnz, mxz - p.z);
-            nx = ny = nz = 0.0f;
+            float nx = 0.0f, ny = 0.0f, nz = 0.0f;
             if (dx <= dy && dx <= dz)
                 nx = p.x - mnx < mxx - p.x ? -1.0f : 1.0f;
             else if (dy <= dz)
                 ny = p.y - mny < mxy - p.y ? -1.0f : 1.0f;
             else
                 nz = p.z - mnz < mxz - p.z ? -1.0f : 1.0f;
-        } else {
-            continue;
+            wall(nx, ny, nz);
         }
-        contact_velocity(nx, ny, nz, vin, restitution, ts, v);
     }
     for (int i = 0; i < c.planeCount; ++i) {
         const pbf_plane pl = c.planes[i];
@@ -443,8 +455,19 @@ __device__ __forceinline__ void resolve_velocity(const float4& p, const float4& 
         const float nz = pl.normal.z * ns;
         const float dist =
             (p.x - pl.point.x) * nx + (p.y - pl.point.y) * ny + (p.z - pl.point.z) * nz;
-        if (dist <= r + eps) contact_velocity(nx, ny, nz, vin, restitution, ts, v);
+        if (dist <= r + eps) wall(nx, ny, nz);
     }
+}
+
+// resolveVelocitiesKernel body (collision_kernels.cu:236-371).
+__device__ __forceinline__ void resolve_velocity(const float4& p, const float4& vin, float4& v,
+                                                 const Colliders& c, float r, float restitution,
+                                                 float friction) {
+#pragma clang fp contract(off)
+    const float ts = 1.0f - friction;
+    for_each_contact(p, c, r, [&](const ContactGeom& g) {
+        contact_velocity(g.nx, g.ny, g.nz, vin, restitution, ts, v);
+    });
 }
 
 // Neighbour loop: kNbrBatch list entries, then their kNbrBatch records, are

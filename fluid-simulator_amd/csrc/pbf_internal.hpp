@@ -60,6 +60,9 @@ struct ListPass {
     // Delta-p: with xperm set, the record x of slot i is at xperm[i] (the
     // grid builds since the last full permutation moved p* only).
     const uint32_t* xperm = nullptr;
+    // Delta-p: with prePos set, also the positions after Delta-p and before
+    // the collision solve (the adjoint's tape).
+    pbf_float4* prePos = nullptr;
 };
 
 // Grid (k_grid.hip)
@@ -181,12 +184,12 @@ struct AdjPost {
     const pbf_float4* omega;  // vorticity, |omega| in .w
     const pbf_float4* vin;    // velocities after gravity
 };
-// One constraint iteration: positions entering it (lambda in .w) and
-// leaving it (after the container projection), in its build's slot order.
+// One constraint iteration: positions entering it (lambda in .w) and after
+// its Delta-p, before the collision solve, in its build's slot order.
 struct AdjIter {
     AdjBuild build;
     const pbf_float4* pk;
-    const pbf_float4* pout;
+    const pbf_float4* pre;
 };
 // Adjoint state.  pb, vb (the adjoints of a substep's output positions and
 // velocities), x0b, v1b (of its input positions and velocities) are in
@@ -210,7 +213,7 @@ struct AdjParams {
     float h, dt, visc, vort, radius, restitution, friction;
     float mass, rho0, lambdaEps, scorrK, scorrDq;
     int scorrN;
-    pbf_container box;
+    pbfdev::Colliders col;  // the container, spheres, boxes and planes (device tables)
 };
 // Post-solve phase backward: from pb (output positions) and vb (output
 // velocities) to x0b, v1b partially and pb (the last iteration's output).

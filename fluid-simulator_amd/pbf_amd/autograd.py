@@ -27,6 +27,12 @@ class Rollout(torch.autograd.Function):
         vel[:, :3] = v0.detach().cpu().numpy()
         solver = PBFSolver()
         solver.initialize(scene.n, scene.minBounds, scene.maxBounds, params)
+        if scene.spheres:
+            solver.setSpheres(scene.spheres)
+        if scene.boxes:
+            solver.setBoxes(scene.boxes)
+        if scene.planes:
+            solver.setPlanes(scene.planes)
         solver.setParticles(scene.positions, vel)
         adj = PBFAdjoint(solver)
         adj.forward(int(frames), int(checkpoint_every))

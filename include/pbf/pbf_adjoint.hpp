@@ -8,10 +8,15 @@
 // then replays each earlier segment from its checkpoint with the tape on and
 // goes back through it, so memory holds one segment's tape at a time.  With
 // checkpointEvery >= frameCount nothing is replayed.  Neighbour lists and every
-// discrete decision of the forward (container projections, wall contacts)
-// are held fixed; between them the derivatives are exact up to FP32
-// rounding.  Models the container only (no spheres, boxes or planes) and
-// needs vorticity confinement enabled (the post-solve pass it records).
+// discrete decision of the forward (which collision projections act and
+// their branches, which contacts act) are held fixed; between them the
+// derivatives are exact up to FP32 rounding.  The collision solve and the
+// contact velocities take the container, spheres, rounded boxes and planes.
+// XSPH viscosity and vorticity strength may be zero (the passes are then
+// skipped, as in the reference, and their derivatives are the one-sided ones
+// at zero); under the adjoint every substep runs the post-solve grid build,
+// which the reference skips when both are zero (the results differ from a
+// plain run only in the summation order of the next substep).
 #pragma once
 
 #include <cstddef>

@@ -506,7 +506,8 @@ pbf_status pbf_batch_kernel_times(pbf_batch* batch, char* names, size_t namesByt
 /* reference: reverse-mode derivatives of a solver's frames, with neighbour */
 /* lists and the forward's discrete decisions held fixed, checkpointed      */
 /* every checkpointEvery frames and replayed with a tape per segment.       */
-/* Container only; vorticity confinement must be enabled.                  */
+/* Container, spheres, rounded boxes and planes; XSPH viscosity and        */
+/* vorticity strength may be zero (see include/pbf/pbf_adjoint.hpp).       */
 /* ------------------------------------------------------------------------ */
 typedef struct pbf_adjoint pbf_adjoint;
 

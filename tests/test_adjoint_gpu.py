@@ -97,6 +97,105 @@ def test_adjoint_matches_fp64_central_differences(gpu, nx, substeps, frames, eve
         assert abs(f - grad) <= 1e-3 * abs(f)
 
 
+# A sphere right of the block, a rounded box inside it and a tilted plane
+# under its left part: the block spreads into the sphere, its particles are
+# pushed out of the box, and the plane lifts the bottom layer, so every
+# collider projects positions and acts on contact velocities within the
+# rollout.  Kept apart from each other and from the walls by more than a
+# particle diameter.
+GPU_COLLIDERS = dict(
+    spheres=(((0.55, 0.15, 0.25), 0.08),),
+    boxes=(((0.25, 0.3, 0.25), (0.05, 0.05, 0.05)),),
+    planes=(((0.0, 0.04, 0.0), (0.05, 1.0, 0.02)),),
+)
+
+
+# Without XSPH (c = 0) the velocities of particles sliding along the
+# tilted plane keep a normal component of rounding size, so whether their
+# contact acts is decided by rounding: over 3-4 frames the FP32 forward and
+# the FP64 restatement then take different branches for a few particles
+# (the FP32 oracle does too).  The c = 0 cases therefore run 2 frames (two
+# checkpoint segments), on a seed whose FP32 oracle stays on the FP64
+# branches (positions within 5e-7).
+@pytest.mark.parametrize("c,e,seed,frames,every", [
+    (1e-4, 1e-4, 9, 4, 2), (1e-4, 0.0, 9, 4, 2), (0.0, 1e-4, 10, 2, 1), (0.0, 0.0, 10, 2, 1)])
+def test_adjoint_with_colliders_matches_fp64_central_differences(gpu, c, e, seed, frames, every):
+    """Spheres, rounded boxes and planes in the collision solve and the
+    contacts (each projects positions and acts on contact velocities within
+    the rollout); XSPH and vorticity confinement each on or off (c = 0 or
+    e = 0: the passes are skipped and the derivatives are those at 0, which
+    central differences of the FP64 forward see: its terms are linear in c
+    and e there)."""
+    from dataclasses import replace
+
+    import pbf_amd
+
+    sc, P, Q, v0, rng = _case(10, 1, seed=seed)
+    P.xsphViscosity, P.vorticityStrength = c, e
+    P.collisionRestitution, P.collisionFriction = 0.2, 0.1
+    Q = replace(Q, restitution=0.2, friction=0.1, **GPU_COLLIDERS)
+    x0 = sc.positions
+    target = x0[:, :3].astype(np.float64) + np.array([0.2, 0.0, 0.0])
+
+    def loss(x):
+        d = x - target
+        return np.mean(np.sum(d * d, 1))
+
+    s = gpu.Solver()
+    s.initialize(sc.n, sc.minBounds, sc.maxBounds, P)
+    s.setSpheres(list(GPU_COLLIDERS["spheres"]))
+    s.setBoxes(list(GPU_COLLIDERS["boxes"]))
+    s.setPlanes(list(GPU_COLLIDERS["planes"]))
+    s.setParticles(x0, v0)
+    adj = pbf_amd.PBFAdjoint(s)
+    adj.forward(frames, every)
+    xT = s.copyPositionsToHost()
+    xbar = np.zeros_like(xT)
+    xbar[:, :3] = (2.0 * (xT[:, :3] - target) / sc.n).astype(np.float32)
+    _, v0b, cb, eb = adj.backward(xbar, None)
+
+    x = x0[:, :3].astype(np.float64)
+    v = v0[:, :3].astype(np.float64)
+    S = frames * P.substeps
+    A_ = __import__("adjoint_ref")
+    kinds, acted, xx, vv = set(), set(), x, v
+    for _ in range(S):
+        xx, vv, aux = A_.forward(xx, vv, lambda p: A_.pairs_within(p, Q.h), Q, c, e,
+                                 keep_aux=True)
+        kinds |= {ev["kind"] for it in aux["it"] for ev in it["events"]}
+        acted |= {cd["kind"] for cd in aux["contacts"] if cd["act"].any()}
+    assert {"sphere", "box", "plane"} <= kinds and {"sphere", "box", "plane"} <= acted
+
+    def L64(v_, c_=c, e_=e):
+        return loss(A_.run_full(x, v_, S, Q, c_, e_)[0])
+
+    base = L64(v)
+    assert abs(base - loss(xT[:, :3].astype(np.float64))) < 1e-5 * base
+    fd, ad = [], []
+    for k in rng.choice(sc.n, 5, replace=False):
+        for d in range(3):
+            vp, vm = v.copy(), v.copy()
+            vp[k, d] += 1e-4
+            vm[k, d] -= 1e-4
+            fd.append((L64(vp) - L64(vm)) / 2e-4)
+            ad.append(float(v0b[k, d]))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    grads = {}
+    for name, base_val, grad in (("xsphViscosity", c, cb), ("vorticityStrength", e, eb)):
+        hh = 1e-5 * (base_val if base_val > 0 else 1e-4)
+        f = ((L64(v, c_=c + hh) - L64(v, c_=c - hh)) if name == "xsphViscosity"
+             else (L64(v, e_=e + hh) - L64(v, e_=e - hh))) / (2 * hh)
+        grads[name] = (f, grad)
+    print(f"[colliders, c {c}, e {e}, {frames} frames] v0: "
+          f"{len(fd)} coordinates, relative error {err:.2e}; "
+          + "; ".join(f"d/d{k} fd {f:+.6e} adjoint {g:+.6e}" for k, (f, g) in grads.items()),
+          flush=True)
+    assert err <= 1e-3
+    for name, (f, g) in grads.items():
+        assert abs(f - g) <= 1e-3 * abs(f), name
+
+
 def test_torch_rollout_gradients_equal_the_adjoint(gpu):
     """torch.autograd through pbf_amd.autograd.rollout hands back exactly
     the adjoint's gradients for a loss built from torch operations."""
