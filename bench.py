@@ -37,9 +37,11 @@ Prints one JSON line:
                  frames after 3 warm-up frames, and its rooflines (--no-8m
                  skips it)
   batched        the batched workload (256 seeded scenes of 2,048-200,000
-                 particles, about 10M in all): one PBFBatchSolver against a
-                 loop of PBFSolver.step() over one solver per scene, and the
-                 speedup (--no-batched skips it)
+                 particles, about 10M in all): one PBFBatchSolver (frames
+                 replayed from HIP graphs, and eager) against a loop of
+                 PBFSolver.step() over one solver per scene (eager, and with
+                 graphs), the speedups, and the share of a batched frame its
+                 kernels occupy (--no-batched skips it)
   adjoint        the differentiable workload (65,536 particles, 60 frames,
                  checkpoints every 10, and one 60-frame segment taped by
                  forward): the adjoint's passes against a plain run of the
@@ -311,12 +313,14 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=No
     report = None
     if tune_frames > 0:
         chosen, table = solver.autotune(tune_frames)
-        report = {"chosen": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock},
+        report = {"chosen": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock,
+                             "particles_per_thread": chosen.particlesPerThread},
                   "frames_per_candidate": tune_frames,
-                  "candidates": [{"neighbor_skin": a, "pass_block": b, "ms_per_frame": ms}
-                                 for a, b, ms in table]}
+                  "candidates": [{"neighbor_skin": a, "pass_block": b, "particles_per_thread": c,
+                                  "ms_per_frame": ms} for a, b, c, ms in table]}
     elif tuning is not None:
-        solver.setTuning(tuning["neighbor_skin"], tuning["pass_block"])
+        solver.setTuning(tuning["neighbor_skin"], tuning["pass_block"],
+                         tuning.get("particles_per_thread", 1))
     stale0 = solver.staleFrames()
     runs = []
     for _ in range(reps):
@@ -347,7 +351,13 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
     """The batched workload (`count` seeded scenes of 2,048-200,000
     particles, shared parameters): one PBFBatchSolver against looping
     PBFSolver.step() over one solver per scene, each frame.  Every scene's
-    result is bitwise the same either way (tests/test_batch_gpu.py)."""
+    result is bitwise the same either way (tests/test_batch_gpu.py,
+    tests/test_full_size_gpu.py).  Both sides run with frames replayed from
+    HIP graphs and launched eagerly; `speedup` is batched (graphs) over the
+    eager loop, the plain single-scene path.  `kernel_busy` is the share of
+    a batched frame its kernels occupy (events around every launch, summed):
+    the launch gaps a persistent per-substep kernel could remove are at most
+    the rest."""
     import numpy as np
 
     scs = scenes.batch_scenes(count)
@@ -360,32 +370,50 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
     b.run(warmup)
     pbf_amd.synchronize()
     chosen, _ = b.autotune(1)
-    t0 = time.perf_counter()
+    tb = {}
+    for graphs in (True, False):
+        b.setGraphs(graphs)
+        b.run(1)  # graph capture out of the timing
+        pbf_amd.synchronize()
+        t0 = time.perf_counter()
+        b.run(frames)
+        pbf_amd.synchronize()
+        tb[graphs] = (time.perf_counter() - t0) / frames
+    b.setProfiling(True)
     b.run(frames)
     pbf_amd.synchronize()
-    tb = (time.perf_counter() - t0) / frames
+    busy = sum(ms for ms, _ in b.kernelTimes().values()) / 1e3 / frames
     stale = b.staleFrames()
     del b
     solvers = []
     for s in scs:
         o = pbf_amd.PBFSolver()
         o.initialize(s.n, s.minBounds, s.maxBounds, s.params)
-        o.setTuning(chosen.neighborSkin, chosen.passBlock)
+        o.setTuning(chosen.neighborSkin, chosen.passBlock, chosen.particlesPerThread)
         o.setParticles(s.positions, s.velocities)
         o.run(warmup)
         solvers.append(o)
-    pbf_amd.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(frames):
+    tl = {}
+    for graphs in (False, True):
         for o in solvers:
-            o.step()
-    tl = (time.perf_counter() - t0) / frames
+            o.setGraphs(graphs)
+            o.step()  # capture
+        pbf_amd.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            for o in solvers:
+                o.step()
+        tl[graphs] = (time.perf_counter() - t0) / frames
     return {"scenes": count, "particles": n, "substeps": sub, "frames": frames,
-            "tuning": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock},
+            "tuning": {"neighbor_skin": chosen.neighborSkin, "pass_block": chosen.passBlock,
+                       "particles_per_thread": chosen.particlesPerThread},
             "stale_frames": stale,
-            "batched_value": n * sub / tb, "looped_value": n * sub / tl,
-            "unit": "particle-steps/s", "batched_ms_per_step": tb * 1e3,
-            "looped_ms_per_step": tl * 1e3, "speedup": tl / tb}
+            "batched_value": n * sub / tb[True], "looped_value": n * sub / tl[False],
+            "unit": "particle-steps/s",
+            "batched_ms_per_step": tb[True] * 1e3, "batched_eager_ms_per_step": tb[False] * 1e3,
+            "looped_ms_per_step": tl[False] * 1e3, "looped_graphs_ms_per_step": tl[True] * 1e3,
+            "speedup": tl[False] / tb[True], "speedup_over_looped_graphs": tl[True] / tb[True],
+            "kernel_busy": busy / tb[False]}
 
 
 def double_dam_break(pbf_amd, scenes, tuning, steps: int = 100, warmup: int = 3) -> dict:

@@ -96,9 +96,11 @@ void write_kernel_times(const std::vector<pbf::PBFSolver::KernelTime>& times, ch
 
 namespace {
 pbf::PBFSolver::Tuning from_c(const pbf_tuning& t) {
-    return pbf::PBFSolver::Tuning{t.neighborSkin, t.passBlock};
+    return pbf::PBFSolver::Tuning{t.neighborSkin, t.passBlock, t.particlesPerThread};
 }
-pbf_tuning to_c(const pbf::PBFSolver::Tuning& t) { return pbf_tuning{t.neighborSkin, t.passBlock}; }
+pbf_tuning to_c(const pbf::PBFSolver::Tuning& t) {
+    return pbf_tuning{t.neighborSkin, t.passBlock, t.particlesPerThread};
+}
 
 template <class Tuner>
 pbf_status run_autotune(Tuner&& tune, const pbf_tuning* candidates, size_t count,

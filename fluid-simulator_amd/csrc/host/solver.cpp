@@ -299,9 +299,11 @@ struct PBFSolver::State {
     // Workgroup size of the neighbour passes (a tuning: results do not
     // depend on it).
     int passBlock = 256;
+    int passPpt = 1;  // particles per thread of the neighbour passes (a tuning)
     pbfint::ListPass lpass() const {
         pbfint::ListPass lp;
         lp.block = passBlock;
+        lp.ppt = passPpt;
         return lp;
     }
     bool skin_active() const {
@@ -1204,7 +1206,8 @@ std::size_t PBFSolver::staleFrames() const { return state_->staleFrames; }
 std::vector<PBFSolver::Tuning> PBFSolver::defaultTunings() {
     std::vector<Tuning> out;
     for (float skin : {0.0f, 0.1f, 0.2f, 0.3f})
-        for (int block : {64, 128, 256}) out.push_back(Tuning{skin, block});
+        for (int block : {64, 128, 256})
+            for (int ppt : {1, 2}) out.push_back(Tuning{skin, block, ppt});
     return out;
 }
 
@@ -1214,6 +1217,8 @@ void check_tuning(const PBFSolver::Tuning& t) {
         throw std::invalid_argument("Neighbour skin must lie in [0, 1]");
     if (t.passBlock < 64 || t.passBlock > 256 || t.passBlock % 64 != 0)
         throw std::invalid_argument("passBlock must be 64, 128, 192 or 256");
+    if (t.particlesPerThread != 1 && t.particlesPerThread != 2 && t.particlesPerThread != 4)
+        throw std::invalid_argument("particlesPerThread must be 1, 2 or 4");
 }
 }  // namespace
 
@@ -1221,10 +1226,13 @@ void PBFSolver::setTuning(const Tuning& tuning) {
     check_tuning(tuning);
     setNeighborSkin(tuning.neighborSkin);
     state_->passBlock = tuning.passBlock;
+    state_->passPpt = tuning.particlesPerThread;
     state_->drop_graphs();
 }
 
-PBFSolver::Tuning PBFSolver::tuning() const { return Tuning{state_->skin, state_->passBlock}; }
+PBFSolver::Tuning PBFSolver::tuning() const {
+    return Tuning{state_->skin, state_->passBlock, state_->passPpt};
+}
 
 std::vector<PBFSolver::TuningTime> PBFSolver::autotune(std::size_t frames,
                                                        const std::vector<Tuning>& candidates) {

@@ -27,6 +27,17 @@ __device__ __forceinline__ size_t thread_index() {
     return static_cast<size_t>(b) * blockDim.x + threadIdx.x;
 }
 
+// Particles per thread (a launch tuning): workgroup b takes the ppt blocks
+// of blockDim particles from b * ppt on, one after another, so the lanes of
+// a wave still touch consecutive particles at every step.  body(i) returns
+// early for particles it skips.
+template <bool kFast, class Body>
+__device__ __forceinline__ void for_each_particle(int ppt, Body body) {
+    const size_t b = kFast ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (int q = 0; q < ppt; ++q)
+        body((b * static_cast<size_t>(ppt) + static_cast<size_t>(q)) * blockDim.x + threadIdx.x);
+}
+
 // Cell key of position p: the scene's own key, offset into the global key
 // range in a batched step (the scene of slot i).
 template <bool kBatched>
@@ -106,8 +117,8 @@ __global__ void __launch_bounds__(kBlock)
     k_lambda_list(const pbf_float4* p, const uint32_t* __restrict__ nbr,
                   const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
                   float eps, float* lambda, size_t lstride, size_t stride,
-                  uint32_t* __restrict__ status, int exactMax) {
-    const size_t i = thread_index<kFast>();
+                  uint32_t* __restrict__ status, int exactMax, int ppt) {
+    for_each_particle<kFast>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (status != nullptr && *status != 0u) return;
     const float4 pi = ld4(p, i);
@@ -147,6 +158,7 @@ __global__ void __launch_bounds__(kBlock)
     const double C = rho / static_cast<double>(rho0) - 1.0;
     lambda[i * lstride] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
     if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
+    });
 }
 
 // computeDeltaPosition (src/pbf/pbf_solver_kernels.cu:108-182).
@@ -256,8 +268,8 @@ __global__ void __launch_bounds__(kBlock)
                      const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                      size_t n, float h, pbf_float4* __restrict__ omega,
                      pbf_float4* __restrict__ pw, size_t stride, uint32_t* __restrict__ status,
-                     int exactMax) {
-    const size_t i = thread_index<kFast>();
+                     int exactMax, int ppt) {
+    for_each_particle<kFast>(ppt, [&](const size_t i) {
     if (i >= n) return;
     const int c = cnt[i];
     const float4 pi = ld4(p, i);
@@ -286,6 +298,7 @@ __global__ void __launch_bounds__(kBlock)
     const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
     if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, wl));
+    });
 }
 
 // applyVorticityConfinement (src/pbf/pbf_solver_kernels.cu:317-385) as a
@@ -569,9 +582,9 @@ __global__ void __launch_bounds__(kBlock)
                           const pbf_float4* p0, float stale2, uint32_t* __restrict__ key,
                           pbf_float3 kmn, float kcs, pbf_int3 kgs, int* __restrict__ clearCells,
                           size_t clearCount, const uint32_t* __restrict__ xperm,
-                          pbf_float4* __restrict__ prePos, uint32_t* __restrict__ status) {
+                          pbf_float4* __restrict__ prePos, uint32_t* __restrict__ status, int ppt) {
     clear_cells(clearCells, clearCount);
-    const size_t i = thread_index<true>();
+    for_each_particle<true>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (*status != 0u) return;
     float4 pi = ld4(pin, i);
@@ -615,6 +628,7 @@ __global__ void __launch_bounds__(kBlock)
         const float ex = pi.x - q.x, ey = pi.y - q.y, ez = pi.z - q.z;
         if (!(ex * ex + ey * ey + ez * ez <= stale2)) atomicOr(status, PBF_STATUS_LIST_STALE);
     }
+    });
 }
 
 // updateVelocityAndPosition + applyXsphViscosity (pbf_solver.cu:284-298).
@@ -630,8 +644,8 @@ __global__ void __launch_bounds__(kBlock)
                   const pbf_float4* __restrict__ vold, pbf_float4* __restrict__ vout,
                   const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                   size_t n, float h, float visc, float inv_dt, size_t stride,
-                  uint32_t* __restrict__ status, int exactMax) {
-    const size_t i = thread_index<true>();
+                  uint32_t* __restrict__ status, int exactMax, int ppt) {
+    for_each_particle<true>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (*status != 0u) return;
     const float4 xi = ld4(x, i);
@@ -670,6 +684,7 @@ __global__ void __launch_bounds__(kBlock)
         });
     st4(vout, i, make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw));
     if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
+    });
 }
 
 // updateVelocityAndPosition without XSPH, velocity only (x <- p* is a swap).
@@ -699,8 +714,8 @@ __global__ void __launch_bounds__(kBlock)
                       const pbf_float4* __restrict__ vcur, const pbf_float4* __restrict__ vsnap,
                       pbf_float4* __restrict__ vout, size_t n, float h, float dt, float strength,
                       size_t stride, Colliders col, float radius, float e, float f,
-                      pbfint::SceneTable st, const uint32_t* __restrict__ status) {
-    const size_t i = thread_index<true>();
+                      pbfint::SceneTable st, const uint32_t* __restrict__ status, int ppt) {
+    for_each_particle<true>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (*status != 0u) return;
     const float4 vi = ld4(vcur, i);
@@ -715,6 +730,7 @@ __global__ void __launch_bounds__(kBlock)
     resolve_velocity(ld4(x, i), ld4(vsnap, i), out, colliders_of<kBatched>(col, i, st), radius, e,
                      f);
     st4(vout, i, out);
+    });
 }
 
 // Clears words with a kernel rather than a memset node, so the step stays
@@ -768,7 +784,7 @@ pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const 
                                 float* lambda, size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_lambda_list<false>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, size_t{1}, stride, static_cast<uint32_t*>(nullptr), 0);
+               rho0, eps, lambda, size_t{1}, stride, static_cast<uint32_t*>(nullptr), 0, 1);
 }
 
 pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
@@ -805,7 +821,7 @@ pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, con
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_vorticity_list<false>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
                omega, static_cast<pbf_float4*>(nullptr), stride, static_cast<uint32_t*>(nullptr),
-               0);
+               0, 1);
 }
 
 pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
@@ -944,10 +960,10 @@ pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* 
                                  size_t stride, const ListPass& lp, uint32_t* status, void* stream) {
     float* lambda = reinterpret_cast<float*>(p) + 3;  // the .w words
     if (lp.p0 != nullptr)
-        PBF_LAUNCH((k_lambda_list<true, true>), grid_for(n, lp.block), lp.block, stream, p, nbr, cnt, maxN, n,
-                   h, mass, rho0, eps, lambda, size_t{4}, stride, status, lp.exactMax);
-    PBF_LAUNCH((k_lambda_list<true, false>), grid_for(n, lp.block), lp.block, stream, p, nbr, cnt, maxN, n, h,
-               mass, rho0, eps, lambda, size_t{4}, stride, status, 0);
+        PBF_LAUNCH((k_lambda_list<true, true>), grid_for(n, lp.block * lp.ppt), lp.block, stream, p, nbr, cnt, maxN, n,
+                   h, mass, rho0, eps, lambda, size_t{4}, stride, status, lp.exactMax, lp.ppt);
+    PBF_LAUNCH((k_lambda_list<true, false>), grid_for(n, lp.block * lp.ppt), lp.block, stream, p, nbr, cnt, maxN, n, h,
+               mass, rho0, eps, lambda, size_t{4}, stride, status, 0, lp.ppt);
 }
 
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
@@ -959,10 +975,10 @@ pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, c
     const bool batched = st.geom != nullptr;
     const bool skinned = lp.p0 != nullptr;
 #define PBF_DELTA(B, S)                                                                     \
-    PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block), lp.block, stream, pin,  \
+    PBF_LAUNCH((k_delta_apply_collide<B, S>), grid_for(n, lp.block * lp.ppt), lp.block, stream, pin,  \
                pout, nbr, cnt, maxN, x, n, h, mass, rho0, sk, sn, sdq, stride, col, radius, \
                st, lp.p0, lp.stale2, lp.keyOut, lp.keyMin, lp.keyCell, lp.keyDims,          \
-               lp.clearCells, lp.clearCount, lp.xperm, lp.prePos, status)
+               lp.clearCells, lp.clearCount, lp.xperm, lp.prePos, status, lp.ppt)
     if (batched && skinned) PBF_DELTA(true, true);
     if (batched) PBF_DELTA(true, false);
     if (skinned) PBF_DELTA(false, true);
@@ -975,10 +991,10 @@ pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pb
                               size_t n, float h, float visc, float inv_dt, size_t stride,
                               const ListPass& lp, uint32_t* status, void* stream) {
     if (lp.exactMax > 0)
-        PBF_LAUNCH(k_update_xsph<true>, grid_for(n, lp.block), lp.block, stream, x, p, vold, vout, nbr, cnt,
-                   maxN, n, h, visc, inv_dt, stride, status, lp.exactMax);
-    PBF_LAUNCH(k_update_xsph<false>, grid_for(n, lp.block), lp.block, stream, x, p, vold, vout, nbr, cnt, maxN,
-               n, h, visc, inv_dt, stride, status, 0);
+        PBF_LAUNCH(k_update_xsph<true>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, p, vold, vout, nbr, cnt,
+                   maxN, n, h, visc, inv_dt, stride, status, lp.exactMax, lp.ppt);
+    PBF_LAUNCH(k_update_xsph<false>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, p, vold, vout, nbr, cnt, maxN,
+               n, h, visc, inv_dt, stride, status, 0, lp.ppt);
 }
 
 pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
@@ -991,8 +1007,8 @@ pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint
                             const int* cnt, int maxN, size_t n, float h, pbf_float4* omega,
                             pbf_float4* pw, size_t stride, const ListPass& lp, uint32_t* status,
                             void* stream) {
-    PBF_LAUNCH(k_vorticity_list<true>, grid_for(n, lp.block), lp.block, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, pw, stride, status, lp.exactMax);
+    PBF_LAUNCH(k_vorticity_list<true>, grid_for(n, lp.block * lp.ppt), lp.block, stream, p, v, nbr, cnt, maxN, n, h,
+               omega, pw, stride, status, lp.exactMax, lp.ppt);
 }
 
 pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,
@@ -1003,11 +1019,12 @@ pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, con
                                   const SceneTable& st, const ListPass& lp,
                                   const uint32_t* status, void* stream) {
     if (st.geom != nullptr)
-        PBF_LAUNCH(k_confine_resolve<true>, grid_for(n, lp.block), lp.block, stream, x, pw, nbr, cnt, maxN,
+        PBF_LAUNCH(k_confine_resolve<true>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, pw, nbr, cnt, maxN,
                    omega, vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st,
-                   status);
-    PBF_LAUNCH(k_confine_resolve<false>, grid_for(n, lp.block), lp.block, stream, x, pw, nbr, cnt, maxN, omega,
-               vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st, status);
+                   status, lp.ppt);
+    PBF_LAUNCH(k_confine_resolve<false>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, pw, nbr, cnt, maxN, omega,
+               vcur, vsnap, vout, n, h, dt, strength, stride, col, radius, e, f, st, status,
+               lp.ppt);
 }
 
 pbf_status launch_resolve_guarded(const pbf_float4* x, const pbf_float4* vsnap, pbf_float4* v,

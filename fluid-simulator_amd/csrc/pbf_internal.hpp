@@ -38,14 +38,15 @@ struct SceneTable {
 // moved more than half the skin from its point (the Delta-p pass checks and
 // raises PBF_STATUS_LIST_STALE); the counts within h replace the counts of
 // the builds they stand in for.
-// A neighbour pass's launch: the skin-list bookkeeping above and the
-// workgroup size (threads, a multiple of 64 up to 256; results do not
-// depend on it).
+// A neighbour pass's launch: the skin-list bookkeeping above, the
+// workgroup size (threads, a multiple of 64 up to 256) and the particles
+// per thread (results depend on neither).
 struct ListPass {
     const pbf_float4* p0 = nullptr;  // predicted positions the lists were built around
     float stale2 = 0.0f;             // squared displacement from their points that is too far
     int exactMax = 0;                // > 0: neighbours within h above this raise the overflow bit
     int block = 256;
+    int ppt = 1;  // particles per thread (a tuning: results do not depend on it)
     // Delta-p: with keyOut set, also the cell keys of its output positions
     // in this grid (the next grid build's keys; a batched step offsets them
     // by the scene table).

@@ -336,28 +336,37 @@ def as_float4(a, n=None) -> np.ndarray:
 # Pinned host memory
 # --------------------------------------------------------------------------
 class Tuning(C.Structure):
-    """pbf_tuning: the skin (cell-size multiplier 1 + neighborSkin) and the
-    workgroup size of the neighbour passes."""
-    _fields_ = [("neighborSkin", C.c_float), ("passBlock", C.c_int)]
+    """pbf_tuning: the skin (cell-size multiplier 1 + neighborSkin), the
+    workgroup size of the neighbour passes and their particles per thread."""
+    _fields_ = [("neighborSkin", C.c_float), ("passBlock", C.c_int),
+                ("particlesPerThread", C.c_int)]
 
     def __repr__(self):
-        return f"Tuning(neighborSkin={self.neighborSkin:.3g}, passBlock={self.passBlock})"
+        return (f"Tuning(neighborSkin={self.neighborSkin:.3g}, passBlock={self.passBlock}, "
+                f"particlesPerThread={self.particlesPerThread})")
 
 
-DEFAULT_TUNING_COUNT = 12
+DEFAULT_TUNING_COUNT = 24
+DEFAULT_TUNINGS = [(s, b, p) for s in (0.0, 0.1, 0.2, 0.3) for b in (64, 128, 256) for p in (1, 2)]
+
+
+def _tuning(t) -> Tuning:
+    """A Tuning from (skin, block) or (skin, block, particles per thread)."""
+    return Tuning(float(t[0]), int(t[1]), int(t[2]) if len(t) > 2 else 1)
 
 
 def _autotune(fn, handle, frames, candidates):
     if candidates:
-        arr = (Tuning * len(candidates))(*[Tuning(float(a), int(b)) for a, b in candidates])
+        arr = (Tuning * len(candidates))(*[_tuning(c) for c in candidates])
         count, ptr = len(candidates), C.cast(arr, _vp)
     else:
         count, ptr = 0, None
     ms = (C.c_double * max(count, DEFAULT_TUNING_COUNT))()
     chosen = Tuning()
     _check(fn(handle, frames, ptr, count, C.cast(ms, _vp), C.cast(C.pointer(chosen), _vp)))
-    tried = candidates or [(s, b) for s in (0.0, 0.1, 0.2, 0.3) for b in (64, 128, 256)]
-    return chosen, [(float(a), int(b), ms[k]) for k, (a, b) in enumerate(tried)]
+    tried = [_tuning(c) for c in (candidates or DEFAULT_TUNINGS)]
+    return chosen, [(t.neighborSkin, t.passBlock, t.particlesPerThread, ms[k])
+                    for k, t in enumerate(tried)]
 
 
 class _PinnedBlock:
@@ -690,8 +699,8 @@ class PBFSolver:
         _check(lib().pbf_solver_neighbor_skin(self._h, C.byref(sk), C.byref(st)))
         return st.value
 
-    def setTuning(self, neighborSkin: float, passBlock: int = 256):
-        t = Tuning(float(neighborSkin), int(passBlock))
+    def setTuning(self, neighborSkin: float, passBlock: int = 256, particlesPerThread: int = 1):
+        t = Tuning(float(neighborSkin), int(passBlock), int(particlesPerThread))
         _check(lib().pbf_solver_set_tuning(self._h, C.cast(C.pointer(t), _vp)))
 
     def tuning(self) -> Tuning:
@@ -700,9 +709,10 @@ class PBFSolver:
         return t
 
     def autotune(self, frames: int, candidates=None):
-        """Times `frames` frames of each (skin, passBlock) candidate (default
-        set when None) from the current state and keeps the fastest.
-        Returns (chosen Tuning, [(skin, passBlock, ms per frame), ...])."""
+        """Times `frames` frames of each (skin, passBlock[, particlesPerThread])
+        candidate (default set when None) from the current state and keeps
+        the fastest.  Returns (chosen Tuning, [(skin, passBlock,
+        particlesPerThread, ms per frame), ...])."""
         return _autotune(lib().pbf_solver_autotune, self._h, frames, candidates)
 
     def setProfiling(self, enabled: bool):
@@ -802,8 +812,8 @@ class PBFBatchSolver:
         _check(lib().pbf_batch_stale_frames(self._h, C.byref(st)))
         return st.value
 
-    def setTuning(self, neighborSkin: float, passBlock: int = 256):
-        t = Tuning(float(neighborSkin), int(passBlock))
+    def setTuning(self, neighborSkin: float, passBlock: int = 256, particlesPerThread: int = 1):
+        t = Tuning(float(neighborSkin), int(passBlock), int(particlesPerThread))
         _check(lib().pbf_batch_set_tuning(self._h, C.cast(C.pointer(t), _vp)))
 
     def autotune(self, frames: int, candidates=None):

@@ -76,6 +76,7 @@ public:
     struct Tuning {
         float neighborSkin = 0.0f;
         int passBlock = 256;
+        int particlesPerThread = 1;  // of the neighbour passes: 1, 2 or 4
     };
     struct TuningTime {
         Tuning tuning;
@@ -85,7 +86,7 @@ public:
     Tuning tuning() const;
     // The autotuner: runs `frames` frames of every candidate (the default
     // set when empty: skins 0, 0.1, 0.2, 0.3 by workgroups of 64, 128,
-    // 256) from the current state, restoring it after each, times them on
+    // 256 by 1 or 2 particles per thread) from the current state, restoring it after each, times them on
     // the host clock (stale frames run again included), and keeps the
     // fastest.  The state, and staleFrames(), are left as they were.
     std::vector<TuningTime> autotune(std::size_t frames,

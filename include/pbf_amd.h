@@ -414,16 +414,18 @@ pbf_status pbf_solver_neighbor_skin(const pbf_solver* solver, float* skin, size_
 
 /* Launch tunings and the autotuner, no counterpart in the reference.  A
  * tuning is the skin (the grid's cell-size multiplier is 1 + neighborSkin,
- * see pbf_solver_set_neighbor_skin) and the workgroup size of the
- * neighbour passes (64, 128, 192 or 256 threads; results do not depend on
- * it). */
+ * see pbf_solver_set_neighbor_skin), the workgroup size of the neighbour
+ * passes (64, 128, 192 or 256 threads) and the particles each of their
+ * threads takes in turn (1, 2 or 4); results depend on neither of the
+ * last two. */
 typedef struct pbf_tuning {
     float neighborSkin;
     int passBlock;
+    int particlesPerThread;
 } pbf_tuning;
 /* Size of the default candidate set: skins 0, 0.1, 0.2, 0.3 by workgroups
- * of 64, 128 and 256 threads. */
-#define PBF_DEFAULT_TUNING_COUNT 12
+ * of 64, 128 and 256 threads by 1 or 2 particles per thread. */
+#define PBF_DEFAULT_TUNING_COUNT 24
 pbf_status pbf_solver_set_tuning(pbf_solver* solver, const pbf_tuning* tuning);
 pbf_status pbf_solver_get_tuning(const pbf_solver* solver, pbf_tuning* tuning);
 /* Runs `frames` frames of each candidate (candidates == NULL: the default
