@@ -147,6 +147,11 @@ pbf_status pbf_grid_build(pbf_grid* grid, const pbf_float4* d_predicted, size_t 
     return guarded([&] { grid->grid.build(d_predicted, n); });
 }
 
+pbf_status pbf_grid_rebuild(pbf_grid* grid, const pbf_float4* d_predicted, size_t n) {
+    PBF_REQUIRE(grid);
+    return guarded([&] { grid->grid.rebuild(d_predicted, n); });
+}
+
 const uint32_t* pbf_grid_sorted_keys(const pbf_grid* g) { return g ? g->grid.sortedKeys() : nullptr; }
 const uint32_t* pbf_grid_sorted_indices(const pbf_grid* g) {
     return g ? g->grid.sortedIndices() : nullptr;
@@ -312,6 +317,11 @@ pbf_status pbf_solver_set_graphs(pbf_solver* s, int enabled) {
     return guarded([&] { s->solver.setGraphs(enabled != 0); });
 }
 
+pbf_status pbf_solver_set_incremental_sort(pbf_solver* s, int enabled) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setIncrementalSort(enabled != 0); });
+}
+
 pbf_status pbf_solver_set_neighbor_skin(pbf_solver* s, float skin) {
     PBF_REQUIRE(s);
     return guarded([&] { s->solver.setNeighborSkin(skin); });
@@ -428,6 +438,11 @@ pbf_status pbf_batch_counts(const pbf_batch* b, size_t* sceneCount, size_t* part
 pbf_status pbf_batch_set_graphs(pbf_batch* b, int enabled) {
     PBF_REQUIRE(b);
     return guarded([&] { b->batch.setGraphs(enabled != 0); });
+}
+
+pbf_status pbf_batch_set_incremental_sort(pbf_batch* b, int enabled) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setIncrementalSort(enabled != 0); });
 }
 
 pbf_status pbf_batch_set_neighbor_skin(pbf_batch* b, float skin) {

@@ -45,8 +45,18 @@ GridCore GridCore::create(std::size_t maxParticles, float3 minBounds, float3 max
     next.sortedIndices.allocate(maxParticles);
     next.cellStart.allocate(next.numCells);
     next.cellEnd.allocate(next.numCells);
-    next.scratch.allocate(pbfint::radix_scratch_words(maxParticles));
+    next.allocateSortScratch();
     return next;
+}
+
+void GridCore::allocateSortScratch() {
+    scratch.allocate(pbfint::grid_scratch_words(maxParticles));
+    scratch.fillBytes(0);  // the re-sort counters start at zero
+    // The memset runs on the null stream, which the solver's non-blocking
+    // stream does not wait for.
+    hip_check(hipDeviceSynchronize());
+    splitters.allocate(pbfint::resort_splitter_count(maxParticles));
+    orderedCount = 0;
 }
 
 int3 GridCore::dimsFor(float3 minBounds, float3 maxBounds, float cellSize, std::size_t* cells) {
@@ -88,17 +98,31 @@ GridCore GridCore::createKeyed(std::size_t maxParticles, std::size_t numCells, f
     next.sortedIndices.allocate(maxParticles);
     next.cellStart.allocate(numCells);
     next.cellEnd.allocate(numCells);
-    next.scratch.allocate(pbfint::radix_scratch_words(maxParticles));
+    next.allocateSortScratch();
     return next;
 }
 
-void GridCore::sortAndRanges(std::size_t n, hipStream_t stream) {
+bool GridCore::sortKeys(std::size_t n, bool resort, hipStream_t stream) {
+    if (resort && n != 0 && orderedCount == n) {
+        orderedCount = 0;  // until the caller's particles follow this sort
+        launch_check(pbfint::resort_pairs(keys.data(), sortedKeys.data(), splitters.data(),
+                                          sortedKeys.data(), sortedIndices.data(), keysTmp.data(),
+                                          indices.data(), n, maxParticles, scratch.data(), stream));
+        return true;
+    }
+    orderedCount = 0;
     launch_check(pbfint::radix_sort_pairs(keys.data(), nullptr, sortedKeys.data(),
                                           sortedIndices.data(), keysTmp.data(), indices.data(), n,
                                           keyBits, scratch.data(), stream));
+    return false;
+}
+
+void GridCore::sortAndRanges(std::size_t n, bool resort, hipStream_t stream) {
+    sortKeys(n, resort, stream);
     launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
     launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,
-                                            dims, nullptr, pbfint::SceneTable{}, stream));
+                                            dims, nullptr, pbfint::SceneTable{}, stream,
+                                            splitters.data()));
 }
 
 SpatialGrid::SpatialGrid() : state_(new State()) {}
@@ -112,7 +136,11 @@ void SpatialGrid::initialize(std::size_t maxParticles, float3 minBounds, float3 
     state_->core = std::move(next);
 }
 
-void SpatialGrid::build(const float4* predicted, std::size_t n) {
+void SpatialGrid::build(const float4* predicted, std::size_t n) { build_impl(predicted, n, false); }
+
+void SpatialGrid::rebuild(const float4* predicted, std::size_t n) { build_impl(predicted, n, true); }
+
+void SpatialGrid::build_impl(const float4* predicted, std::size_t n, bool resort) {
     GridCore& g = state_->core;
     if (g.maxParticles == 0) throw std::logic_error("SpatialGrid must be initialized before building");
     if (predicted == nullptr && n != 0)
@@ -124,14 +152,17 @@ void SpatialGrid::build(const float4* predicted, std::size_t n) {
         // An empty build still clears the ranges of a previous build.
         launch_check(pbfint::launch_fill_cells(g.cellStart.data(), g.cellEnd.data(), g.numCells,
                                                stream));
+        g.orderedCount = 0;
         hip_check(hipDeviceSynchronize());
         return;
     }
     launch_check(pbfint::launch_keys(predicted, g.keys.data(), n, g.minBounds, g.cellSize, g.dims,
                                      pbfint::SceneTable{}, stream));
-    g.sortAndRanges(n, stream);
+    g.sortAndRanges(n, resort, stream);
     // Surface asynchronous failures from this call (spatial_grid.cu:220-225).
     hip_check(hipDeviceSynchronize());
+    // Positions laid out in this build's order may be re-sorted from it.
+    g.orderedCount = n;
 }
 
 const std::uint32_t* SpatialGrid::sortedKeys() const { return state_->core.sortedKeys.data(); }

@@ -38,7 +38,14 @@ struct GridCore {
     DeviceBuffer<std::uint32_t> sortedIndices;
     DeviceBuffer<int> cellStart;
     DeviceBuffer<int> cellEnd;
-    DeviceBuffer<std::uint32_t> scratch;        // radix histograms and scan partials
+    DeviceBuffer<std::uint32_t> scratch;        // sort histograms, scan partials, re-sort counters
+    // The last sort's keys at multiples of the re-sort tile, written with
+    // its cell ranges.
+    DeviceBuffer<std::uint32_t> splitters;
+    // Particle count of the last sort when the caller's particles are in
+    // its output order (0: not known to be): a sort of that many keys may
+    // then re-sort from it.
+    std::size_t orderedCount = 0;
 
     // Validates like the reference and allocates; throws on failure.
     static GridCore create(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
@@ -49,9 +56,17 @@ struct GridCore {
     // The keyed part of a grid for a batch of scenes: numCells global cells
     // (the scenes' grids laid end to end), no bounds of its own.
     static GridCore createKeyed(std::size_t maxParticles, std::size_t numCells, float cellSize);
+    // scratch (zeroed) and splitters for maxParticles keys.
+    void allocateSortScratch();
 
-    // Stable sort of `keys` (slot values implicit) and the cell ranges.
-    void sortAndRanges(std::size_t particleCount, hipStream_t stream);
+    // Stable sort of `keys` (slot values implicit) into sortedKeys and
+    // sortedIndices: re-sorted from the last sort when `resort` is set and
+    // orderedCount == particleCount, else radix-sorted; the results are the
+    // same bit for bit.  Leaves orderedCount at 0 (the caller sets it once
+    // its particles follow the new order).  Returns whether it re-sorted.
+    bool sortKeys(std::size_t particleCount, bool resort, hipStream_t stream);
+    // sortKeys, then the cell ranges (and the splitters).
+    void sortAndRanges(std::size_t particleCount, bool resort, hipStream_t stream);
 };
 
 struct SpatialGrid::State {

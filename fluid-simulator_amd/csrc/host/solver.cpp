@@ -300,6 +300,15 @@ struct PBFSolver::State {
     // depend on it).
     int passBlock = 256;
     int passPpt = 1;  // particles per thread of the neighbour passes (a tuning)
+    // Grid builds re-sort from the previous build when the particles are in
+    // its order (the same result as a full sort, bit for bit).
+    bool resortBuilds = true;
+    // Nothing is known of the order of the particles (new particles, a
+    // restored checkpoint): the next builds sort in full.
+    void forget_order() {
+        grid.orderedCount = 0;
+        sgrid.orderedCount = 0;
+    }
     pbfint::ListPass lpass() const {
         pbfint::ListPass lp;
         lp.block = passBlock;
@@ -331,9 +340,20 @@ struct PBFSolver::State {
     struct FrameGraph {
         BufferKey start{};
         bool skin = false;               // a frame with skin lists
+        // Whether the particles start (and end) in the order of the grid's
+        // and the skin grid's last sort: the first build re-sorts or not.
+        std::array<bool, 2> orderedIn{}, orderedOut{};
         std::array<int, kSlots> perm{};  // slot k ends up holding slot perm[k]'s buffer
         hipGraphExec_t exec = nullptr;
     };
+    std::array<bool, 2> ordered() const {
+        return {count != 0 && grid.orderedCount == count,
+                count != 0 && sgrid.orderedCount == count};
+    }
+    void set_ordered(const std::array<bool, 2>& o) {
+        grid.orderedCount = o[0] ? count : 0;
+        sgrid.orderedCount = o[1] ? count : 0;
+    }
     std::vector<FrameGraph> graphs;
     bool useGraphs = false;
 
@@ -413,16 +433,19 @@ struct PBFSolver::State {
         }
         const BufferKey key = buffer_key();
         const bool sk = skin_active();
+        const std::array<bool, 2> ord = ordered();
         for (const FrameGraph& g : graphs) {
-            if (g.start == key && g.skin == sk) {
+            if (g.start == key && g.skin == sk && g.orderedIn == ord) {
                 hip_check(hipGraphLaunch(g.exec, stream));
                 permute(g.perm);
+                set_ordered(g.orderedOut);
                 return;
             }
         }
         FrameGraph fg;
         fg.start = key;
         fg.skin = sk;
+        fg.orderedIn = ord;
         hipGraph_t graph = nullptr;
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -432,11 +455,14 @@ struct PBFSolver::State {
             if (graph != nullptr) (void)hipGraphDestroy(graph);
             permute(perm_between(buffer_key(), key));
             recordsLag = false;  // nothing ran: every record is where the frame found it
+            set_ordered(ord);
             throw;
         }
         const hipError_t ce = hipStreamEndCapture(stream, &graph);
         fg.perm = perm_between(key, buffer_key());
+        fg.orderedOut = ordered();
         permute(perm_between(buffer_key(), key));  // nothing ran yet
+        set_ordered(ord);
         if (ce != hipSuccess) {
             if (graph != nullptr) (void)hipGraphDestroy(graph);
             hip_check(ce);
@@ -447,6 +473,7 @@ struct PBFSolver::State {
         graphs.push_back(fg);
         hip_check(hipGraphLaunch(fg.exec, stream));
         permute(fg.perm);
+        set_ordered(fg.orderedOut);
     }
 
     void ensure_stream() {
@@ -459,16 +486,18 @@ struct PBFSolver::State {
     // already in grid.keys and cellStart is already cleared: the prediction
     // kernel does both for a substep's first build, each Delta-p pass for
     // the build after it.
+    //
+    // The particles sit in the order of this grid's last sort when that
+    // sort's build was the last one (orderedCount): the keys are then
+    // re-sorted from it, which gives the radix sort's result bit for bit.
     void build_and_reorder(bool sk = false, bool deferRecords = false) {
         const std::size_t n = count;
         GridCore& grid = grid_of(sk);
+        grid_of(!sk).orderedCount = 0;  // its order is about to be lost
         {
-            Scope sc(prof, "radix_sort", stream);
-            launch_check(pbfint::radix_sort_pairs(grid.keys.data(), nullptr,
-                                                  grid.sortedKeys.data(),
-                                                  grid.sortedIndices.data(), grid.keysTmp.data(),
-                                                  grid.indices.data(), n, grid.keyBits,
-                                                  grid.scratch.data(), stream));
+            const bool re = resortBuilds && grid.orderedCount == n;
+            Scope sc(prof, re ? "resort" : "radix_sort", stream);
+            grid.sortKeys(n, re, stream);
         }
         {
             Scope sc(prof, "ranges_gather", stream);
@@ -486,8 +515,9 @@ struct PBFSolver::State {
             launch_check(pbfint::launch_ranges_gather(grid.sortedKeys.data(),
                                                       grid.cellStart.data(), grid.cellEnd.data(),
                                                       n, grid.dims, tiles_of(sk), scenes(sk), pm,
-                                                      stream));
+                                                      stream, grid.splitters.data()));
         }
+        grid.orderedCount = n;  // p* (and what follows it) is in the new order
         std::swap(p, pa);
         if (deferRecords) {
             std::swap(perm, perma);
@@ -865,6 +895,7 @@ struct PBFSolver::State {
                                      hipMemcpyDeviceToDevice, stream));
             hip_check(hipMemcpyAsync(id.data(), ckid.data(), n * sizeof(std::uint32_t),
                                      hipMemcpyDeviceToDevice, stream));
+            forget_order();
             ++staleFrames;
             reset_status();
             forceExact = true;
@@ -1096,6 +1127,7 @@ void PBFSolver::setParticles(const float4* positions, const float4* velocities, 
     std::swap(s.x, s.xa);
     std::swap(s.v, s.va);
     s.count = n;
+    s.forget_order();
 }
 
 void PBFSolver::setParticlesDevice(const float4* positions, const float4* velocities,
@@ -1119,6 +1151,7 @@ void PBFSolver::setParticlesDevice(const float4* positions, const float4* veloci
     launch_check(pbf_k_initialize_stable_particle_ids(s.id.data(), n, s.stream));
     hip_check(hipStreamSynchronize(s.stream));
     s.count = n;
+    s.forget_order();
 }
 
 void PBFSolver::setSpheres(const std::vector<SphereCollider>& spheres) {
@@ -1262,6 +1295,7 @@ std::vector<PBFSolver::TuningTime> PBFSolver::autotune(std::size_t frames,
         copy(s.x.data(), sx.data(), sizeof(float4));
         copy(s.v.data(), sv.data(), sizeof(float4));
         copy(s.id.data(), sid.data(), sizeof(std::uint32_t));
+        s.forget_order();
         hip_check(hipStreamSynchronize(s.stream));
     };
     try {
@@ -1295,6 +1329,11 @@ std::vector<PBFSolver::TuningTime> PBFSolver::autotune(std::size_t frames,
 void PBFSolver::setGraphs(bool enabled) {
     state_->useGraphs = enabled;
     if (!enabled) state_->drop_graphs();
+}
+
+void PBFSolver::setIncrementalSort(bool enabled) {
+    state_->resortBuilds = enabled;
+    state_->drop_graphs();
 }
 
 void PBFSolver::setProfiling(bool enabled) {
@@ -1446,6 +1485,7 @@ std::size_t PBFBatchSolver::sceneFirst(std::size_t scene) const {
 }
 
 void PBFBatchSolver::setGraphs(bool enabled) { solver_.setGraphs(enabled); }
+void PBFBatchSolver::setIncrementalSort(bool enabled) { solver_.setIncrementalSort(enabled); }
 void PBFBatchSolver::setNeighborSkin(float skin) { solver_.setNeighborSkin(skin); }
 std::size_t PBFBatchSolver::staleFrames() const { return solver_.staleFrames(); }
 void PBFBatchSolver::setTuning(const PBFSolver::Tuning& tuning) { solver_.setTuning(tuning); }
@@ -1510,6 +1550,7 @@ struct PBFAdjoint::Impl {
         copy(s.x, c.x, n * sizeof(float4));
         copy(s.v, c.v, n * sizeof(float4));
         copy(s.id, c.id, n * sizeof(std::uint32_t));
+        s.forget_order();
     }
 };
 

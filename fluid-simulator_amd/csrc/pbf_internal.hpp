@@ -80,10 +80,11 @@ pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
 // Cell ranges of the sorted keys; with tileEnd not null, also the end of
 // every non-empty sweep tile (sweep_tile_count(dims) entries, only those of
 // non-empty tiles written).
+// With splitters not null, also the keys at multiples of the re-sort tile
+// (resort_splitter_count(n) entries) for the next resort_pairs.
 pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                              uint32_t* tileEnd, const SceneTable& st, void* stream);
-// The working-set permutation of a grid build: outputs xo..ido at slot i
-// take the records of slot idx[i].
+                              uint32_t* tileEnd, const SceneTable& st, void* stream,
+                              uint32_t* splitters = nullptr);
 // The working-set permutation of a grid build: slot i takes the records of
 // slot idx[i].  With x null only p* moves, and permOut[i] = permIn[idx[i]]
 // (idx[i] without permIn) records where the other records of slot i still
@@ -101,8 +102,19 @@ struct Permute {
 // launch_cell_ranges (tileEnd required) and the permutation in one pass.
 pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
                                 uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
-                                void* stream);
+                                void* stream, uint32_t* splitters = nullptr);
 size_t radix_scratch_words(size_t n);
+// Scratch of a grid of capacity n: the radix sort's words, then the
+// re-sort's counters (which must be zero before the first resort_pairs).
+size_t grid_scratch_words(size_t n);
+size_t resort_splitter_count(size_t n);
+// Stable sort of (keys, slot) into kout/vout, equal to radix_sort_pairs',
+// from prev, the previous sort's output keys of the same slots, and its
+// splitters; fast when few keys changed.  kout may be prev; tmpK/tmpV are
+// distinct from the rest; scratch holds grid_scratch_words(capacity).
+pbf_status resort_pairs(const uint32_t* keys, const uint32_t* prev, const uint32_t* splitters,
+                        uint32_t* kout, uint32_t* vout, uint32_t* tmpK, uint32_t* tmpV, size_t n,
+                        size_t capacity, uint32_t* scratch, void* stream);
 int radix_key_bits(size_t numCells);
 // Stable sort of (key, value) pairs by the low `bits` key bits into
 // kout/vout (vin == nullptr: values are the slots); the five arrays are

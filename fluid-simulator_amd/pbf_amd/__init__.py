@@ -211,6 +211,7 @@ _SIGNATURES = {
     "pbf_grid_destroy": (_st, [_vp]),
     "pbf_grid_initialize": (_st, [_vp, _sz, float3, float3, _f]),
     "pbf_grid_build": (_st, [_vp, _vp, _sz]),
+    "pbf_grid_rebuild": (_st, [_vp, _vp, _sz]),
     "pbf_grid_build_static": (_st, [_vp, _sz, float3, float3, _f, _vp, _vp, _vp, _vp, _sz, _vp]),
     "pbf_grid_sorted_keys": (_vp, [_vp]),
     "pbf_grid_sorted_indices": (_vp, [_vp]),
@@ -248,6 +249,7 @@ _SIGNATURES = {
     "pbf_solver_stream": (_vp, [_vp]),
     "pbf_solver_set_profiling": (_st, [_vp, _i]),
     "pbf_solver_set_graphs": (_st, [_vp, _i]),
+    "pbf_solver_set_incremental_sort": (_st, [_vp, _i]),
     "pbf_solver_set_neighbor_skin": (_st, [_vp, C.c_float]),
     "pbf_solver_neighbor_skin": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(_sz)]),
     "pbf_solver_set_tuning": (_st, [_vp, _vp]),
@@ -271,6 +273,7 @@ _SIGNATURES = {
     "pbf_batch_copy_velocities_to_host": (_st, [_vp, _vp, _sz]),
     "pbf_batch_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
     "pbf_batch_set_graphs": (_st, [_vp, _i]),
+    "pbf_batch_set_incremental_sort": (_st, [_vp, _i]),
     "pbf_batch_set_neighbor_skin": (_st, [_vp, C.c_float]),
     "pbf_batch_stale_frames": (_st, [_vp, C.POINTER(_sz)]),
     "pbf_batch_set_tuning": (_st, [_vp, _vp]),
@@ -479,6 +482,11 @@ class SpatialGrid:
     def build(self, predictedPositions, particleCount):
         _check(lib().pbf_grid_build(self._h, _ptr(predictedPositions), int(particleCount)))
 
+    def rebuild(self, predictedPositions, particleCount):
+        """build's results, re-sorted from the last build (fast when the
+        positions are its particles in its sorted order, moved a little)."""
+        _check(lib().pbf_grid_rebuild(self._h, _ptr(predictedPositions), int(particleCount)))
+
     def _info(self):
         g, nc, mp, cs, mn, mx = int3(), _sz(), _sz(), _f(), float3(), float3()
         _check(lib().pbf_grid_info(self._h, C.byref(g), C.byref(nc), C.byref(mp), C.byref(cs),
@@ -683,6 +691,11 @@ class PBFSolver:
         """Replay frames from captured HIP graphs, or launch eagerly (default)."""
         _check(lib().pbf_solver_set_graphs(self._h, 1 if enabled else 0))
 
+    def setIncrementalSort(self, enabled: bool):
+        """Grid builds by re-sort from the previous build (default) or by
+        full radix sort; the results are bitwise the same."""
+        _check(lib().pbf_solver_set_incremental_sort(self._h, 1 if enabled else 0))
+
     def setNeighborSkin(self, skin: float):
         """Skin lists: one grid build per substep, radius h * (1 + skin); 0 runs
         the reference's build before every iteration and after the solve."""
@@ -803,6 +816,9 @@ class PBFBatchSolver:
 
     def setGraphs(self, enabled: bool):
         _check(lib().pbf_batch_set_graphs(self._h, 1 if enabled else 0))
+
+    def setIncrementalSort(self, enabled: bool):
+        _check(lib().pbf_batch_set_incremental_sort(self._h, 1 if enabled else 0))
 
     def setNeighborSkin(self, skin: float):
         _check(lib().pbf_batch_set_neighbor_skin(self._h, float(skin)))

@@ -53,6 +53,7 @@ public:
     std::size_t sceneFirst(std::size_t scene) const;
 
     void setGraphs(bool enabled);
+    void setIncrementalSort(bool enabled);  // PBFSolver::setIncrementalSort
     // Skin lists for every scene (PBFSolver::setNeighborSkin).  A frame in
     // which any scene's lists go stale runs again for every scene with the
     // reference's builds.

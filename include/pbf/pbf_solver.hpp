@@ -58,6 +58,10 @@ public:
     // Replay frames from captured HIP graphs (default off; see
     // pbf_solver_set_graphs).  No counterpart in the reference.
     void setGraphs(bool enabled);
+    // Grid builds re-sort from the previous build when the particles are in
+    // its order (default on; see pbf_solver_set_incremental_sort).  Results
+    // are bitwise the same either way.  No counterpart in the reference.
+    void setIncrementalSort(bool enabled);
     // Skin lists: one grid build per substep instead of one per constraint
     // iteration plus the post-solve rebuild (see
     // pbf_solver_set_neighbor_skin).  skin is a fraction of the smoothing

@@ -27,6 +27,12 @@ public:
 
     // Keys, stable sort and cell ranges of device positions; synchronises.
     void build(const float4* predictedPositions, std::size_t particleCount);
+    // build's results bit for bit, computed by re-sorting from the last
+    // build: fast when the positions are that build's particles laid out in
+    // its sorted order (slot i holding particle sortedIndices()[i]) after a
+    // small move, so that few cell keys changed.  Without a previous build
+    // of particleCount particles it is build.  No reference counterpart.
+    void rebuild(const float4* predictedPositions, std::size_t particleCount);
 
     const std::uint32_t* sortedKeys() const;
     const std::uint32_t* sortedIndices() const;
@@ -43,6 +49,7 @@ public:
     struct State;  // library-internal
 
 private:
+    void build_impl(const float4* predictedPositions, std::size_t particleCount, bool resort);
     std::unique_ptr<State> state_;
 };
 

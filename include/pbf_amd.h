@@ -275,6 +275,15 @@ pbf_status pbf_grid_initialize(pbf_grid* grid, size_t maxParticles, pbf_float3 m
 /* SpatialGrid::build (spatial_grid.cu:139-226): keys, stable sort, ranges.
  * Synchronises before returning, like the reference. */
 pbf_status pbf_grid_build(pbf_grid* grid, const pbf_float4* d_predicted, size_t particleCount);
+/* pbf_grid_build's results bit for bit, computed by re-sorting from the
+ * grid's last build.  No counterpart in the reference: it is for callers
+ * that lay their particles out in the last build's order (slot i holding
+ * particle pbf_grid_sorted_indices()[i], as the solver's reorder does) and
+ * build again after a small move; then only the keys that changed are
+ * sorted and merged into the kept ones.  Correct for any positions; without
+ * a previous build of particleCount particles it is pbf_grid_build.
+ * Synchronises before returning. */
+pbf_status pbf_grid_rebuild(pbf_grid* grid, const pbf_float4* d_predicted, size_t particleCount);
 /* Entry point that allocates nothing.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
@@ -393,6 +402,15 @@ void* pbf_solver_stream(pbf_solver* solver);
  * eagerly while profiling is on. */
 pbf_status pbf_solver_set_graphs(pbf_solver* solver, int enabled);
 
+/* Grid builds by re-sort (enabled = 1, the default) or by full radix sort
+ * (0).  No counterpart in the reference, which sorts every build's keys in
+ * full (spatial_grid.cu:205-213).  The step's builds find the particles in
+ * the order of the previous build's sort, and only a few per cent of their
+ * cell keys change from one build to the next: those are sorted on their
+ * own and merged into the kept ones (pbf_grid_rebuild).  Results are
+ * bitwise the same either way; the option exists to compare the two. */
+pbf_status pbf_solver_set_incremental_sort(pbf_solver* solver, int enabled);
+
 /* Skin lists (Verlet lists), no counterpart in the reference.  With skin > 0
  * a substep builds the grid and the neighbour lists once, instead of before
  * every constraint iteration and after the solve (pbf_solver.cu:210-282):
@@ -489,6 +507,7 @@ pbf_status pbf_batch_copy_velocities_to_host(pbf_batch* batch, pbf_float4* h_out
                                              size_t particleCount);
 pbf_status pbf_batch_counts(const pbf_batch* batch, size_t* sceneCount, size_t* particleCount);
 pbf_status pbf_batch_set_graphs(pbf_batch* batch, int enabled);
+pbf_status pbf_batch_set_incremental_sort(pbf_batch* batch, int enabled);
 /* Skin lists for every scene of the batch (pbf_solver_set_neighbor_skin).
  * A frame in which any scene's lists go stale runs again for all scenes
  * with the reference's builds. */

@@ -64,6 +64,12 @@ class _GpuGrid:
         self.g.build(d, len(p))
         return self.g.read(len(p))
 
+    def rebuild(self, positions):
+        p = f4(positions)
+        d = pbf_amd.DeviceArray.from_host(p, 16)
+        self.g.rebuild(d, len(p))
+        return self.g.read(len(p))
+
 
 class _OracleCollision:
     """CollisionSystem numerics through the oracle (validation is the library's)."""
