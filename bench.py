@@ -32,6 +32,10 @@ Prints one JSON line:
                  five list builds per substep, as the reference)
   b_alg_roofline the same with the compulsory-byte model of one build per
                  step and no stored lists (356 + 56 I + 16 ceil(b/8) + 8 C/N)
+  grid_sort      the same frames with every grid build radix-sorted in full
+                 instead of re-sorted from the previous build (bitwise the
+                 same results): ms per frame and sort microseconds per build
+                 of both (--no-sort-comparison skips it)
   double_dam_break_8m  the 8,388,608-particle double dam break (3 substeps
                  per frame) with the same tuning: particle-steps/s over 10
                  frames after 3 warm-up frames, and its rooflines (--no-8m
@@ -109,6 +113,16 @@ PHASE_BYTES = {
 }
 # The two kinds of build permutation (bytes per particle, as above).
 RANGES_GATHER_DEFERRED, RANGES_GATHER_FULL = 48, 148
+# A build's re-sort from the previous build (k_resort_*): keys and previous
+# keys read (8), keys read again by the merge (4), sorted keys and slots
+# written (8); per changed key its segment (written and read, 8) and its
+# (key, slot) entry (written and read, 16).
+RESORT_BYTES, RESORT_CHANGED_BYTES = 20, 24
+# Keys that change per build on the 1M dam break at frame 60 (FP32 oracle):
+# 20.8% at a substep's first build, 4.3-4.6% at the other four, 7.7% over
+# the five.
+RESORT_CHANGED = 0.077
+RESORT_PER_PARTICLE = RESORT_BYTES + RESORT_CHANGED_BYTES * RESORT_CHANGED
 # Flop model: per neighbour of each pass (the FP32 operations of the
 # reference's expression trees) and 8 per candidate radius test.
 FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
@@ -127,6 +141,8 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int,
     passes uses it instead of the count at rest."""
     if kernel == "radix_sort":
         return 16.0 * ((key_bits + 7) // 8) * n
+    if kernel == "resort":
+        return RESORT_PER_PARTICLE * n
     if kernel == "cell_ranges":
         return 4.0 * n + 8.0 * cells
     if kernel == "fill_cells":
@@ -139,14 +155,17 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int,
     return float(b) * n
 
 
-def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int) -> float:
+def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int,
+                            resort: bool = True) -> float:
     """Algorithmic bytes of one substep of this dataflow per particle: the
     reference's five grid builds (one per constraint iteration plus the
-    post-solve rebuild), each a sort, the ranges, the permute and the
+    post-solve rebuild), each a sort (a re-sort from the previous build, or
+    with resort=False the radix sort), the ranges, the permute and the
     neighbour lists; the keys come from the prediction and the Delta-p
     passes."""
     builds = iterations + 1
-    build = 16 * ((key_bits + 7) // 8) + 4 * cells / n + PHASE_BYTES["build_lists"]
+    sort = RESORT_PER_PARTICLE if resort else 16 * ((key_bits + 7) // 8)
+    build = sort + 4 * cells / n + PHASE_BYTES["build_lists"]
     permute = RANGES_GATHER_DEFERRED * iterations + RANGES_GATHER_FULL
     return (PHASE_BYTES["predict_key"] + builds * build + permute
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
@@ -295,7 +314,7 @@ def copy_calibration(nbytes: int) -> int:
 
 
 def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=None,
-               tune_frames: int = 0, reps: int = 1, count_every: int = 0):
+               tune_frames: int = 0, reps: int = 1, count_every: int = 0, resort: bool = True):
     """Warm-up frames, then (tune_frames > 0) the library's autotuner from
     the warmed-up state (it restores that state), or the given tuning, then
     `reps` runs of `steps` timed frames, each started from the warmed-up
@@ -303,10 +322,12 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=No
     every run computes the same frames).  With count_every > 0 the last run
     also reads the neighbour counts every count_every frames (outside its
     timing: that run is then not timed).  Returns (seconds per run, kernel
-    times, tuning report, mean neighbour count or None)."""
+    times, tuning report, mean neighbour count or None).  resort=False sorts
+    every grid build in full (the results are bitwise the same)."""
     solver = pbf_amd.PBFSolver()
     solver.initialize(scene.n, scene.minBounds, scene.maxBounds, scene.params)
     solver.setParticles(scene.positions, scene.velocities)
+    solver.setIncrementalSort(resort)
     solver.run(warmup)
     pbf_amd.synchronize()
     x0, v0 = solver.copyPositionsToHost(), solver.copyVelocitiesToHost()
@@ -345,6 +366,23 @@ def run_frames(pbf_amd, scene, warmup: int, steps: int, profile: bool, tuning=No
             counts.append(float(solver.neighborLists(0)[1].mean()))
         kmean = statistics.fmean(counts)
     return runs, times, report, kmean
+
+
+def sort_comparison(pbf_amd, scene, warmup: int, steps: int, tuning, resort_s: float,
+                    resort_rows: dict) -> dict:
+    """The same frames with every grid build radix-sorted in full (the
+    results are bitwise the same): milliseconds per frame of both, and the
+    sort's microseconds per build (events around the launches)."""
+    runs, _, _, _ = run_frames(pbf_amd, scene, warmup, steps, profile=False, tuning=tuning,
+                               resort=False)
+    _, times, _, _ = run_frames(pbf_amd, scene, warmup, steps, profile=True, tuning=tuning,
+                                resort=False)
+    radix = times.get("radix_sort", (0.0, 0))
+    re = resort_rows.get("resort", {})
+    return {"resort_ms_per_step": resort_s * 1e3 / steps, "radix_ms_per_step": runs[0] * 1e3 / steps,
+            "resort_us_per_build": re.get("avg_ms", 0.0) * 1e3 if re else None,
+            "radix_us_per_build": radix[0] / max(radix[1], 1) * 1e3,
+            "builds": radix[1], "resort_bytes_per_particle": RESORT_PER_PARTICLE}
 
 
 def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2) -> dict:
@@ -509,6 +547,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-8m", action="store_true",
                     help="skip the 8M double dam break measured after the 1M workload")
+    ap.add_argument("--no-sort-comparison", action="store_true",
+                    help="skip the frames with every grid build radix-sorted")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default tuning (the reference's builds, 256-thread passes)")
     ap.add_argument("--tune-frames", type=int, default=3,
@@ -594,6 +634,11 @@ def main() -> int:
                 valu["issue"] = {"insts_per_launch": insts, "issue_ms": issue_ms,
                                  "frac": issue_ms / drow["avg_ms"]}
 
+    sort_cmp = None
+    if not a.no_profile_pass and not a.no_sort_comparison:
+        sort_cmp = sort_comparison(pbf_amd, scene, a.warmup, a.steps,
+                                   tuning["chosen"] if tuning else None, elapsed, kernel_rows)
+
     bstep = step_bytes_per_particle(scene.params.solverIterations, key_bits, cells, n)
     step_roof = {"bytes_per_particle_step": bstep,
                  "achieved": bstep * value / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -655,6 +700,7 @@ def main() -> int:
                                            value=headline_8m["value"])
                                       if headline_8m and "value" in headline_8m else None),
         "double_dam_break_8m": headline_8m,
+        "grid_sort": sort_cmp,
         "batched": batched,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
