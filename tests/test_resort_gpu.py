@@ -11,14 +11,15 @@ std::stable_sort expectation of test/test_spatial_grid.cu:77-142
     in a row, at particle counts around the 1,024-slot tile (1, 1,023,
     1,024, 1,025) and at 300,000;
   * no key changed (the merge copies the kept slots) and every key changed;
-  * one segment far above the LDS capacity of 2,048 entries (most
+  * one segment far above the LDS capacity of 1,024 entries (most
     particles jump above the previous largest key), which sorts that
     segment in global memory;
   * positions not in the previous build's order at all (correct, only
     slower) and a changed particle count (falls back to the full sort);
   * the solver and the batched solver step bitwise the same with the
     re-sort on and off, eagerly and from graphs, with and without skin
-    lists.
+    lists;
+  * pbf_grid_rebuild validates its arguments like pbf_grid_build.
 """
 from __future__ import annotations
 
@@ -84,7 +85,7 @@ def test_rebuild_with_a_segment_beyond_lds(gpu):
     # The previous build holds every particle in the lowest quarter of the
     # box; 80% of them then jump to the top, above every previous key, so
     # all those entries fall into the last tile's segment (about 32,000
-    # entries against an LDS capacity of 2,048).
+    # entries against an LDS capacity of 1,024).
     rng = np.random.default_rng(11)
     n = 40_000
     g = gpu.grid(n, MIN, MAX, CS)
@@ -163,3 +164,37 @@ def test_batch_steps_equal_with_resort_on_and_off(gpu):
         out.append((b.copyPositionsToHost(), b.copyVelocitiesToHost()))
     assert np.array_equal(_bits(out[0][0]), _bits(out[1][0]))
     assert np.array_equal(_bits(out[0][1]), _bits(out[1][1]))
+
+
+def test_rebuild_validates_like_build(gpu):
+    """pbf_grid_rebuild raises what pbf_grid_build raises (spatial_grid.cu:
+    139-160): before initialize, on null positions, above capacity; a grid
+    initialised again (another capacity) forgets the old order."""
+    import pbf_amd
+
+    g = pbf_amd.SpatialGrid()
+    with pytest.raises(pbf_amd.LogicError):
+        g.rebuild(None, 1)
+    g.initialize(1000, MIN, MAX, CS)
+    with pytest.raises(pbf_amd.InvalidArgument):
+        g.rebuild(None, 1)
+    rng = np.random.default_rng(3)
+    p = _box(1000, rng)
+    d = pbf_amd.DeviceArray.from_host(p, 16)
+    with pytest.raises(pbf_amd.OutOfRange):
+        g.rebuild(d, 1001)
+    g.rebuild(d, 1000)
+    sk, si, cs, ce = g.read(1000)
+    _check((sk, si, cs, ce), p)
+    g.initialize(2000, MIN, MAX, CS)  # a new grid: no previous order
+    q = _box(2000, rng)
+    dq = pbf_amd.DeviceArray.from_host(q, 16)
+    g.rebuild(dq, 2000)
+    _check(g.read(2000), q)
+    r = _move(q[g.read(2000)[1]], rng)
+    dr = pbf_amd.DeviceArray.from_host(r, 16)
+    g.rebuild(dr, 2000)
+    _check(g.read(2000), r)
+    g.rebuild(dr, 0)  # an empty build clears the ranges, as build does
+    _, _, cs0, ce0 = g.read(0)
+    assert np.all(cs0 == -1) and np.all(ce0 == -1)
