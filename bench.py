@@ -385,7 +385,8 @@ def sort_comparison(pbf_amd, scene, warmup: int, steps: int, tuning, resort_s: f
             "builds": radix[1], "resort_bytes_per_particle": RESORT_PER_PARTICLE}
 
 
-def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2) -> dict:
+def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2,
+                      looped: bool = True) -> dict:
     """The batched workload (`count` seeded scenes of 2,048-200,000
     particles, shared parameters): one PBFBatchSolver against looping
     PBFSolver.step() over one solver per scene, each frame.  Every scene's
@@ -423,6 +424,11 @@ def batched_vs_looped(pbf_amd, scenes, count: int, frames: int, warmup: int = 2)
     busy = sum(ms for ms, _ in b.kernelTimes().values()) / 1e3 / frames
     stale = b.staleFrames()
     del b
+    if not looped:
+        return {"scenes": count, "particles": n, "substeps": sub, "frames": frames,
+                "batched_value": n * sub / tb[True], "unit": "particle-steps/s",
+                "batched_ms_per_step": tb[True] * 1e3,
+                "batched_eager_ms_per_step": tb[False] * 1e3, "kernel_busy": busy / tb[False]}
     solvers = []
     for s in scs:
         o = pbf_amd.PBFSolver()
@@ -542,6 +548,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-versus-looped measurement (256 scenes)")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-looped", action="store_true",
+                    help="batched leg without the loop of per-scene solvers")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (65,536 particles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
@@ -659,7 +667,8 @@ def main() -> int:
 
     batched = None
     if not a.no_batched:
-        batched = batched_vs_looped(pbf_amd, scenes, a.batch_scenes, frames=3)
+        batched = batched_vs_looped(pbf_amd, scenes, a.batch_scenes, frames=3,
+                                    looped=not a.no_looped)
 
     adjoint = None if a.no_adjoint else adjoint_cost(pbf_amd, scenes)
 
