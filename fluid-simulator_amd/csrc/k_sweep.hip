@@ -1,4 +1,4 @@
-// Neighbour lists of the solver step by LDS-tiled cell sweep (gfx950).
+// Neighbour lists of the solver step by LDS-staged cell sweep (gfx950).
 //
 // Builds exactly the list findNeighbors builds (src/pbf/neighbors.cu:3-91):
 // for particle i, every other particle j of the 3x3x3 cells around i's cell
@@ -7,29 +7,34 @@
 // overflow bit above maxN).  The reference stores it slot-major at
 // nbr[q * stride + i]; this build stores it in groups of four (below).
 // The reference reads every candidate through global memory per particle;
-// here the candidates are staged in LDS per tile:
+// here each wave stages its particles' candidates in LDS:
 //
-//   * a tile is up to 32 consecutive cells of one grid row (fixed y, z).
-//     Particles are sorted by cell key (x fastest), so the cells
-//     x0-1 .. x1+1 of any row are one contiguous range of the working set:
-//     a tile's neighbourhood is 9 such row segments (dz, dy in -1..1);
-//   * sorted keys visit tiles in increasing order; workgroup b takes the
-//     tiles whose first particle lies in [256 b, 256 b + 256), so no empty
-//     tile is visited and no list of tiles is built;
-//   * the 9 segments are staged one z-plane (3 rows) at a time, coordinates
-//     in separate arrays, with coalesced loads;
-//   * each thread owns one particle of the tile.  For every row it tests the
-//     candidates of cells cx-1 .. cx+1 in windows of 32 (branch-free, four
-//     loads in flight) and appends the set bits, lowest first, to its list;
+//   * a wave takes 64 consecutive particles of the sorted working set, one
+//     per lane, and works on its own (no workgroup barrier): every lane is
+//     busy whatever the cell occupancy;
+//   * particles are sorted by cell key (x fastest), so the wave's particles
+//     of one grid row (a "piece") occupy cells xa .. xb of that row, and for
+//     every (dz, dy) the cells xa-1 .. xb+1 of the row at (y+dy, z+dz) are
+//     one contiguous run of the working set (a segment).  A wave holds at
+//     most two pieces of at most kMaxWidth cells each; any other wave (spray,
+//     sparse surface rows) gives every lane a sweep of its own through
+//     global memory;
+//   * per (piece, dz, dy) sixteen lanes read the segment's cell ranges and
+//     scan them into per-cell offsets in LDS;
+//   * row by row (dz, dy), the wave stages both pieces' segments in LDS as
+//     packed (x, y, z) and each lane tests the candidates of its cells
+//     cx-1 .. cx+1 in windows of 32 (four compares per step folded into a
+//     32-bit hit mask, candidates in order from the top bit down);
+//   * the set bits of a window are appended four per pass through a per-lane
+//     LDS column that holds the open group of four: the pass writes four
+//     entries after the open ones, and a filled group leaves as one 16-byte
+//     store.  The wave runs as many passes as its busiest lane needs quarter
+//     hits, not hits;
 //   * lists are stored in groups of four entries, group k of particle i at
 //     nbr[k * stride + i] (16 bytes), so one store carries four entries and
 //     the list kernels read four entries with one load.  Slots of the last
 //     group past the count repeat its last entry, so every slot of a
-//     written group is a valid index;
-//   * a workgroup whose range starts more than kMaxTilePasses tiles (sparse
-//     regions: spray, surface sheets) gives each of their particles a lane
-//     of its own that reads its 27 cells from global memory instead of
-//     sweeping the tiles one after another.
+//     written group is a valid index.
 //
 // Requires a 3x3x3 stencil (cell size >= the list radius), which the
 // solver's grids guarantee (src/pbf/pbf_solver.cu:81-82 for the reference's
@@ -46,28 +51,28 @@ using namespace pbfdev;
 
 namespace {
 
-constexpr int kTileW = static_cast<int>(pbfint::kSweepTileW);  // cells per tile along x
-constexpr int kSegCells = kTileW + 2;  // tile cells plus the x halo
-constexpr int kSegs = 9;               // (dz, dy) rows, dz-major
-constexpr int kThreads = 256;
-constexpr int kCap = 1536;             // staged particles per z-plane
-constexpr int kPad = 4;                // test reads run up to 3 records past a row
-constexpr int kMaxTilePasses = 4;      // more tiles per workgroup take the lane path
-constexpr int kStageUnroll = 3;        // staging loads in flight per thread
+constexpr int kWaves = 4;                  // independent waves per workgroup
+constexpr int kThreads = 64 * kWaves;
+constexpr int kSegCells = 16;              // cells of a segment: one scan group
+constexpr int kMaxWidth = kSegCells - 2;   // cells a piece may span
+constexpr int kPieces = 2;                 // grid rows a staged wave may span
+constexpr int kSegs = 9;                   // (dz, dy) rows, dz-major
+constexpr int kStage = 192;                // staged records per row (both pieces)
+constexpr int kPad = 4;                    // test reads run up to 3 records past a window
+constexpr int kStageUnroll = 3;            // staging loads in flight per lane
+constexpr int kBufRows = 12;               // per-lane append column (see append)
 
 struct Grid {
     const pbf_float4* p;
     const int* cellStart;
     const int* cellEnd;
     const uint32_t* sortedKeys;
-    const uint32_t* tileEnd;
     uint4* nbr;  // grouped lists
     int* cnt;
     uint32_t* status;
     size_t n;
     size_t stride;
     pbf_int3 dims;
-    int chunks;  // tiles per row = ceil(dims.x / kTileW)
     float h2;
     int maxNeighbors;
     pbfint::SceneTable st;  // a batched step's scenes
@@ -79,24 +84,20 @@ struct Grid {
     uint32_t fullBit;
 };
 
-// The grid a tile or particle belongs to: the one grid of a single-scene
-// step, or its scene's grid in a batched step (keys and tiles offset into
-// the global ranges).
+// The grid a particle belongs to: the one grid of a single-scene step, or
+// its scene's grid in a batched step (keys offset into the global range).
 struct Geom {
     pbf_int3 dims;
     uint32_t cellOff;
-    uint32_t tileOff;
-    int chunks;
     pbf_container box;  // the scene's container (skin lists)
 };
 
 template <bool kBatched>
 __device__ __forceinline__ Geom geom_of(const Grid& g, size_t slot) {
-    if (!kBatched) return Geom{g.dims, 0u, 0u, g.chunks, g.box};
+    if (!kBatched) return Geom{g.dims, 0u, g.box};
     const uint32_t scene = g.st.sceneOf[slot];
     const pbfint::SceneGeom& sg = g.st.geom[scene];
-    return Geom{sg.dims, sg.cellOff, sg.tileOff, static_cast<int>(sg.chunks),
-                g.skin ? g.st.containers[scene] : pbf_container{}};
+    return Geom{sg.dims, sg.cellOff, g.skin ? g.st.containers[scene] : pbf_container{}};
 }
 
 // The position a candidate is tested at: its own, or for skin lists its
@@ -107,59 +108,43 @@ __device__ __forceinline__ float3 test_point(const pbf_container& box, float rad
     return make_float3(p.x, p.y, p.z);
 }
 
-struct Meta {
-    int segFirst[kSegs];  // working-set index of the segment's first particle
-    int segBase[kSegs];   // LDS slot of that particle within its plane
-    int segCells[kSegs];  // cells of the segment (0: empty or outside the grid)
-    int planeCount[3];
-    int cellOff[kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
-    int stop;
-    int waveTiles[kThreads / 64];
-    int tiles[kThreads];      // tiles starting in this workgroup's particle range
-    int tileFirst[kThreads];  // their first particles
-};
-
-// Staged coordinates of one z-plane as packed 12-byte (x, y, z) records:
-// the twelve loads of a four-candidate step share one address register with
-// immediate offsets, and lanes ~8 records apart (neighbouring cells) land
-// 24 banks apart, where 16-byte records would share one bank group.
-struct Plane {
-    float xyz[3 * (kCap + kPad)];
+// One wave's LDS: the staged row, the segment offsets, and the lanes' append
+// columns.  Coordinates are packed 12-byte (x, y, z) records: the twelve
+// loads of a four-candidate step share one address register with immediate
+// offsets, and lanes ~8 records apart (neighbouring cells) land 24 banks
+// apart, where 16-byte records would share one bank group.
+struct WaveLds {
+    float xyz[3 * (kStage + kPad)];
+    int off[kPieces][kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
+    int first[kPieces][kSegs];               // working-set index of the segment's start
+    uint32_t buf[kBufRows][64];              // row r of lane l at buf[r][l]
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
 // on XCD b % 8).  Give every XCD one contiguous run of particle ranges so
-// neighbouring tiles, which share halo rows, share that XCD's L2.
+// neighbouring waves, which share halo rows, share that XCD's L2.
 __device__ __forceinline__ int xcd_task(int b, int nb) {
     return static_cast<int>(xcd_block(static_cast<unsigned>(b), static_cast<unsigned>(nb)));
 }
 
-// Workgroup barrier for LDS hand-offs only.  No thread of the sweep reads
-// global memory that another thread of the same launch writes, so the
-// barrier orders LDS alone and leaves global loads in flight across it
-// (__syncthreads would wait for every outstanding global load first).
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+// Orders one wave's LDS accesses for the compiler.  A wave's LDS operations
+// execute in order, so what one lane writes is visible to every lane's later
+// reads without a workgroup barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t tile_of(uint32_t key, const Geom& gm) {
-    const uint32_t k = key - gm.cellOff;
-    const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
-    return gm.tileOff + (k / gx) * static_cast<uint32_t>(gm.chunks) +
-           (k % gx) / static_cast<uint32_t>(kTileW);
-}
-
-// Candidates of one row segment: staged in LDS, or read from the working
-// set when the plane did not fit.  Index q is relative to the segment's
-// first particle; position() may be asked for up to 3 records past the row
+// Candidates of one segment: staged in LDS, or read from the working set
+// when the row did not fit.  Index q is relative to the segment's first
+// particle; position() may be asked for up to 3 records past a window
 // (padding in LDS, clamped in global memory).
 struct LdsRow {
-    const Plane* pl;
+    const float* xyz;
     int base;
     __device__ float3 position(int q) const {
-        const float* r = pl->xyz + 3 * (base + q);
+        const float* r = xyz + 3 * (base + q);
         return make_float3(r[0], r[1], r[2]);
     }
 };
@@ -198,23 +183,73 @@ __device__ __forceinline__ uint32_t push4(uint32_t mask, float h2, float d0, flo
     return mask;
 }
 
-// A particle's list under construction.  buf holds the open group of four,
-// newest entry in .w; count is the number of neighbours found (it may pass
-// maxNeighbors), stored the number kept (at most maxNeighbors).
+// A particle's list under construction: the next group's address, the
+// neighbours found (may pass maxNeighbors), the entries kept (at most
+// maxNeighbors), and how many of them wait in the open group (rows 4 ..
+// 4 + open - 1 of the lane's append column).
 struct ListState {
-    uint4 buf;
+    uint4* out;
     int count;
     int stored;
+    int open;
 };
 
-// Appends the neighbours of particle gi among candidates [lo, hi) of one
-// row (first = the row's first working-set index) to gi's list, in order.
+// Appends the set bits of a window's mask (candidate b at bit 31 - (lz of
+// b), read from the top down, entry base + lz) to the list.
+//
+// Each pass takes the next four set bits (the entries past the lane's last
+// are garbage and land beyond its open group), writes them to rows
+// 4 + open .. 7 + open of the lane's column, and when the open group (rows
+// 4 .. 7) fills, stores it and writes the four again at rows open .. open+3:
+// the entries past the stored group then sit at rows 4 .. in order, and the
+// rest falls in the spare rows 0 .. 3.
+__device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t base,
+                                       ListState& st, WaveLds& L, int lane) {
+    int hits = __builtin_popcount(mask);
+    st.count += hits;
+    const int room = g.maxNeighbors - st.stored;
+    if (hits > room) {
+        // Entries past maxNeighbors are counted, not stored: keep the first
+        // room-many candidates (the highest bits).
+        for (int k = hits - room; k > 0; --k) mask &= mask - 1u;
+        hits = room;
+    }
+    for (int rem = hits; rem > 0; rem -= 4) {
+        uint32_t e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int lz = __builtin_clzg(mask, -1);
+            e[u] = base + static_cast<uint32_t>(lz);
+            mask &= 0x7fffffffu >> (lz & 31);
+        }
+        uint32_t* col = &L.buf[4 + st.open][lane];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) col[64 * u] = e[u];
+        const int taken = min(rem, 4);
+        st.stored += taken;
+        const int open = st.open + taken;
+        if (open >= 4) {
+            *st.out = make_uint4(L.buf[4][lane], L.buf[5][lane], L.buf[6][lane], L.buf[7][lane]);
+            st.out += g.stride;
+            uint32_t* low = &L.buf[st.open][lane];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) low[64 * u] = e[u];
+            st.open = open - 4;
+        } else {
+            st.open = open;
+        }
+    }
+}
+
+// Appends the neighbours of the particle at test point ip among candidates
+// [lo, hi) of one segment (first = the segment's first working-set index;
+// selfq = the particle's own index in the segment, or -1) to its list.
 template <class Src>
 __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 ip, int lo, int hi,
-                                          int selfq, size_t gi, size_t first, ListState& st) {
+                                          int selfq, size_t first, ListState& st, WaveLds& L,
+                                          int lane) {
     for (int c = lo; c < hi; c += 32) {
         const int w = min(32, hi - c);
-        const int w4 = (w + 3) & ~3;
         uint32_t mask = 0u;
         for (int b = 0; b < w; b += 4) {
             float3 j[4];
@@ -226,209 +261,44 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
                 d[u] = dist2_exact(ip.x - j[u].x, ip.y - j[u].y, ip.z - j[u].z);
             mask = push4(mask, g.h2, d[0], d[1], d[2], d[3]);
         }
-        // Candidate b sits at bit w4 - 1 - b; reverse so that it sits at b.
-        mask = __builtin_bitreverse32(mask) >> (32 - w4);
-        if (w < 32) mask &= (1u << w) - 1u;
-        if (selfq >= c && selfq < c + 32) mask &= ~(1u << (selfq - c));
+        // Candidate b sits at bit w4 - 1 - b: drop the step's padding
+        // candidates (b >= w, the lowest bits) and the particle itself.
+        const int w4 = (w + 3) & ~3;
+        mask &= ~0u << (w4 - w);
+        const int sq = selfq - c;
+        if (static_cast<unsigned>(sq) < static_cast<unsigned>(w)) mask &= ~(1u << (w4 - 1 - sq));
         if (mask == 0u) continue;
-        // Entries past maxNeighbors are counted, not stored: keep the
-        // lowest free-slot-many bits (rare), so the loop below keeps all.
-        const int hits = __builtin_popcount(mask);
-        const int room = g.maxNeighbors - st.stored;
-        st.count += hits;
-        if (hits > room) {
-            uint32_t kept = 0u;
-            for (int k = 0; k < room; ++k) {
-                const uint32_t low = mask & (0u - mask);
-                kept |= low;
-                mask ^= low;
-            }
-            mask = kept;
-        }
-        uint4* out = g.nbr + static_cast<size_t>(st.stored >> 2) * g.stride + gi;
-        const uint32_t base = static_cast<uint32_t>(first) + static_cast<uint32_t>(c);
-        while (mask != 0u) {
-            const uint32_t e = base + static_cast<uint32_t>(__builtin_ctz(mask));
-            mask &= mask - 1u;
-            st.buf = make_uint4(st.buf.y, st.buf.z, st.buf.w, e);
-            if ((++st.stored & 3) == 0) {
-                *out = st.buf;
-                out += g.stride;
-            }
-        }
+        append(g, mask, static_cast<uint32_t>(first) + static_cast<uint32_t>(c + w4 - 32), st, L,
+               lane);
     }
 }
 
 // Stores the open group, its free slots repeating the last entry, and the
 // count.
-__device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const ListState& st) {
-    const int k = st.stored & 3;
-    if (k != 0) {
-        const uint4 b = st.buf;
-        const uint4 last = k == 1   ? make_uint4(b.w, b.w, b.w, b.w)
-                           : k == 2 ? make_uint4(b.z, b.w, b.w, b.w)
-                                    : make_uint4(b.y, b.z, b.w, b.w);
-        g.nbr[static_cast<size_t>(st.stored >> 2) * g.stride + gi] = last;
+__device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const ListState& st,
+                                                const WaveLds& L, int lane) {
+    if (st.open > 0) {
+        const uint32_t a = L.buf[4][lane];
+        const uint32_t b = st.open > 1 ? L.buf[5][lane] : a;
+        const uint32_t c = st.open > 2 ? L.buf[6][lane] : b;
+        *st.out = make_uint4(a, b, c, c);
     }
     g.cnt[gi] = st.count;
     if (st.count > g.maxNeighbors) atomicOr(g.status, g.fullBit);
 }
 
-// One tile: segment geometry, then the tile's particles against the three
-// z-planes of its halo.  Every branch that reaches a barrier is uniform
-// over the workgroup (it depends on LDS values written before a barrier).
-template <bool kSkin>
-__device__ __forceinline__ void sweep_tile(const Grid& g, const Geom& gm, Meta& m, Plane& lds,
-                                           int tile) {
-    const int local = tile - static_cast<int>(gm.tileOff);
-    const int row = local / gm.chunks;
-    const int chunk = local - row * gm.chunks;
-    const int y = row % gm.dims.y;
-    const int z = row / gm.dims.y;
-    const int x0 = chunk * kTileW;
-    const int x1 = min(x0 + kTileW, gm.dims.x) - 1;
-    const int xs = max(x0 - 1, 0);
-    const int xe = min(x1 + 1, gm.dims.x - 1);
-    const int ncell = xe - xs + 1;
-    const int tid = static_cast<int>(threadIdx.x);
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-
-    // Segment geometry: one wave per row, one lane per cell.
-    for (int s = wave; s < kSegs; s += kThreads / 64) {
-        const int yy = y + (s % 3) - 1;
-        const int zz = z + (s / 3) - 1;
-        const bool rowOk = yy >= 0 && yy < gm.dims.y && zz >= 0 && zz < gm.dims.z;
-        int cnt = 0;
-        int st = 0x7fffffff;
-        if (rowOk && lane < ncell) {
-            const uint32_t key = gm.cellOff + key_of(xs + lane, yy, zz, gm.dims);
-            const int a = g.cellStart[key];
-            if (a != -1) {
-                cnt = g.cellEnd[key] - a;
-                st = a;
-            }
-        }
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
-        int first = st;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) first = min(first, __shfl_xor(first, o, 64));
-        if (lane <= ncell) m.cellOff[s][lane] = incl - cnt;
-        if (lane == 63) {
-            m.segFirst[s] = incl > 0 ? first : 0;
-            m.segCells[s] = incl > 0 ? ncell : 0;
-        }
-    }
-    lds_barrier();
-    if (tid < 3) {
-        int base = 0;
-        for (int r = 0; r < 3; ++r) {
-            const int s = 3 * tid + r;
-            m.segBase[s] = base;
-            base += m.segCells[s] > 0 ? m.cellOff[s][m.segCells[s]] : 0;
-        }
-        m.planeCount[tid] = base;
-    }
-    lds_barrier();
-
-    const int c0 = m.cellOff[4][x0 - xs];
-    const int c1 = m.cellOff[4][x1 - xs + 1];
-    const size_t own0 = static_cast<size_t>(m.segFirst[4]);
-
-    // Usually one round; more when the tile holds more particles than threads.
-    for (int r0 = c0; r0 < c1; r0 += kThreads) {
-        const int r = r0 + tid;
-        const bool active = r < c1;
-        const size_t gi = own0 + static_cast<size_t>(active ? r : c0);
-        float3 ip = make_float3(0.0f, 0.0f, 0.0f);
-        int mc = 0;
-        ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
-        if (active) {
-            ip = test_point<kSkin>(gm.box, g.radius, ld4(g.p, gi));
-            mc = static_cast<int>((g.sortedKeys[gi] - gm.cellOff) %
-                                  static_cast<uint32_t>(gm.dims.x)) - xs;
-        }
-        for (int pl = 0; pl < 3; ++pl) {
-            const int pc = m.planeCount[pl];
-            if (pc == 0) continue;
-            const bool staged = pc <= kCap;
-            if (staged) {
-                lds_barrier();  // the previous plane's readers are done
-                // The plane's three segments are consecutive in LDS: record q
-                // of [0, pc) is row 0 below segBase of row 1, row 1 below
-                // segBase of row 2, else row 2 (an empty row has no q).  A
-                // thread issues kStageUnroll loads before the first store.
-                const int s0 = 3 * pl;
-                const int b1 = m.segBase[s0 + 1];
-                const int b2 = m.segBase[s0 + 2];
-                const size_t f0 = static_cast<size_t>(m.segFirst[s0]);
-                const size_t f1 = static_cast<size_t>(m.segFirst[s0 + 1]) - b1;
-                const size_t f2 = static_cast<size_t>(m.segFirst[s0 + 2]) - b2;
-                for (int q0 = tid; q0 < pc; q0 += kStageUnroll * kThreads) {
-                    float4 v[kStageUnroll];
-#pragma unroll
-                    for (int u = 0; u < kStageUnroll; ++u) {
-                        const int q = q0 + u * kThreads;
-                        if (q < pc) v[u] = ld4(g.p, (q < b1 ? f0 : (q < b2 ? f1 : f2)) + q);
-                    }
-#pragma unroll
-                    for (int u = 0; u < kStageUnroll; ++u) {
-                        const int q = q0 + u * kThreads;
-                        if (q < pc) {
-                            const float3 a = test_point<kSkin>(gm.box, g.radius, v[u]);
-                            float* r = lds.xyz + 3 * q;
-                            r[0] = a.x;
-                            r[1] = a.y;
-                            r[2] = a.z;
-                        }
-                    }
-                }
-                lds_barrier();
-            }
-            if (!active) continue;
-            for (int rr = 0; rr < 3; ++rr) {
-                const int s = 3 * pl + rr;
-                const int nc = m.segCells[s];
-                if (nc == 0) continue;
-                const int lo = m.cellOff[s][max(mc - 1, 0)];
-                const int hi = m.cellOff[s][min(mc + 1, nc - 1) + 1];
-                if (hi <= lo) continue;
-                const int selfq = s == 4 ? r : -1;
-                const size_t first = static_cast<size_t>(m.segFirst[s]);
-                if (staged) {
-                    const LdsRow src{&lds, m.segBase[s]};
-                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
-                } else {
-                    const GlobalRow<kSkin> src{g.p, first, m.cellOff[s][nc] - 1, gm.box,
-                                               g.radius};
-                    sweep_row(g, src, ip, lo, hi, selfq, gi, first, st);
-                }
-            }
-        }
-        if (active) finish_particle(g, gi, st);
-    }
-    lds_barrier();  // LDS is reused by the next tile
-}
-
 // One particle on one lane, candidates read from global memory: the row
 // range of every (dz, dy) is the run of particles from the first non-empty
-// cell of cx-1 .. cx+1 to the last, as in the tile form.
-template <bool kBatched, bool kSkin>
-__device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
-    const Geom gm = geom_of<kBatched>(g, gi);
-    const uint32_t key = g.sortedKeys[gi] - gm.cellOff;
+// cell of cx-1 .. cx+1 to the last.
+template <bool kSkin>
+__device__ __forceinline__ void sweep_lane(const Grid& g, const Geom& gm, size_t gi, uint32_t key,
+                                           ListState& st, WaveLds& L, int lane) {
     const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
     const int cx = static_cast<int>(key % gx);
     const uint32_t rowId = key / gx;
     const int cy = static_cast<int>(rowId % static_cast<uint32_t>(gm.dims.y));
     const int cz = static_cast<int>(rowId / static_cast<uint32_t>(gm.dims.y));
     const float3 ip = test_point<kSkin>(gm.box, g.radius, ld4(g.p, gi));
-    ListState st{make_uint4(0u, 0u, 0u, 0u), 0, 0};
     const int xa = max(cx - 1, 0);
     const int xb = min(cx + 1, gm.dims.x - 1);
     for (int s = 0; s < kSegs; ++s) {
@@ -446,68 +316,167 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, size_t gi) {
         if (rowStart < 0) continue;
         const int len = rowEnd - rowStart;
         const int selfq = s == 4 ? static_cast<int>(gi - static_cast<size_t>(rowStart)) : -1;
-        const GlobalRow<kSkin> src{g.p, static_cast<size_t>(rowStart), len - 1, gm.box,
-                                   g.radius};
-        sweep_row(g, src, ip, 0, len, selfq, gi, static_cast<size_t>(rowStart), st);
+        const GlobalRow<kSkin> src{g.p, static_cast<size_t>(rowStart), len - 1, gm.box, g.radius};
+        sweep_row(g, src, ip, 0, len, selfq, static_cast<size_t>(rowStart), st, L, lane);
     }
-    finish_particle(g, gi, st);
 }
 
-// One workgroup per 256 sorted particles: it sweeps every tile whose first
-// particle falls in its range (none, one, or several small ones).  Seven
-// waves per SIMD (72 VGPRs, three spilled on cold paths) rather than the six
-// that 77 VGPRs allow: the sweep waits on LDS and the list stores most of
-// the time, and the seventh wave hides part of it.
+// One wave per 64 sorted particles (four independent waves per workgroup).
+// Six workgroups fit a CU's LDS (6.6 KB per wave).
 template <bool kBatched, bool kSkin>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_build_lists(Grid g) {
-    __shared__ Meta m;
-    __shared__ Plane lds;
+    __shared__ WaveLds ldsAll[kWaves];
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int wv = static_cast<int>(threadIdx.x) >> 6;
+    WaveLds& L = ldsAll[wv];
+    const size_t w0 = (static_cast<size_t>(xcd_task(static_cast<int>(blockIdx.x),
+                                                    static_cast<int>(gridDim.x))) *
+                           kWaves + static_cast<size_t>(wv)) * 64;
+    if (w0 >= g.n) return;
+    // A raised status word stops the step.
+    if (__builtin_amdgcn_readfirstlane(*g.status) != 0u) return;
 
-    const int tid = static_cast<int>(threadIdx.x);
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const size_t i = static_cast<size_t>(xcd_task(static_cast<int>(blockIdx.x),
-                                                  static_cast<int>(gridDim.x))) *
-                         kThreads + static_cast<size_t>(tid);
-    bool starts = false;
-    uint32_t t = 0;
-    if (i < g.n) {
-        t = tile_of(g.sortedKeys[i], geom_of<kBatched>(g, i));
-        starts = i == 0 || tile_of(g.sortedKeys[i - 1], geom_of<kBatched>(g, i - 1)) != t;
+    const size_t gi = w0 + static_cast<size_t>(lane);
+    const bool active = gi < g.n;
+    const size_t gc = active ? gi : g.n - 1;
+    const int lastLane = static_cast<int>(min(g.n - 1 - w0, static_cast<size_t>(63)));
+    const Geom gm = geom_of<kBatched>(g, gc);
+    const uint32_t skey = g.sortedKeys[gc];
+    const uint32_t key = skey - gm.cellOff;
+    const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
+    const int cx = static_cast<int>(key % gx);
+    const uint32_t rowId = key / gx;
+    const int cy = static_cast<int>(rowId % static_cast<uint32_t>(gm.dims.y));
+    const int cz = static_cast<int>(rowId / static_cast<uint32_t>(gm.dims.y));
+    // The key of cell x = 0 of the particle's row names the row across scenes.
+    const uint32_t rowKey = skey - static_cast<uint32_t>(cx);
+
+    // Pieces: runs of lanes in one grid row.
+    const uint32_t prevRow = __shfl_up(rowKey, 1, 64);
+    const uint64_t starts = __ballot(lane == 0 || prevRow != rowKey);
+    const int pieces = __popcll(starts);
+    const int split = pieces > 1 ? __builtin_ctzll(starts & ~1ull) : 64;
+    const int first1 = min(split, 63);
+    const int xa0 = __shfl(cx, 0, 64), xb0 = __shfl(cx, min(split - 1, lastLane), 64);
+    const int xa1 = __shfl(cx, first1, 64), xb1 = __shfl(cx, lastLane, 64);
+    const bool wide = xb0 - xa0 >= kMaxWidth || (pieces > 1 && xb1 - xa1 >= kMaxWidth);
+
+    ListState st{g.nbr + gi, 0, 0, 0};
+    if (pieces > kPieces || wide) {
+        if (active) {
+            sweep_lane<kSkin>(g, gm, gi, key, st, L, lane);
+            finish_particle(g, gi, st, L, lane);
+        }
+        return;
     }
-    // Tiles in particle order: wave counts, then ranks within the wave.
-    const uint64_t ballot = __ballot(starts);
-    if (lane == 0) m.waveTiles[wave] = __popcll(ballot);
-    // A raised status word stops the step; read once, through LDS, so the
-    // whole workgroup takes the decision together.
-    if (tid == 0) m.stop = *g.status != 0u ? 1 : 0;
-    lds_barrier();
-    if (m.stop != 0) return;
-    int before = 0, total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) {
-        before += w < wave ? m.waveTiles[w] : 0;
-        total += m.waveTiles[w];
+
+    // Segment offsets: sixteen lanes per (piece, row), one cell each.
+    const int pairs = pieces * kSegs;
+    for (int t0 = 0; t0 < pairs; t0 += 4) {
+        const int t = t0 + (lane >> 4);
+        const int k = lane & 15;
+        const int p = t >= kSegs ? 1 : 0;
+        const int s = t - p * kSegs;
+        const int src = p != 0 ? first1 : 0;
+        const int pcy = __shfl(cy, src, 64);
+        const int pcz = __shfl(cz, src, 64);
+        const Geom pg = kBatched ? geom_of<kBatched>(g, w0 + static_cast<size_t>(src)) : gm;
+        const int xs = max((p != 0 ? xa1 : xa0) - 1, 0);
+        const int xe = min((p != 0 ? xb1 : xb0) + 1, pg.dims.x - 1);
+        const int yy = pcy + (s % 3) - 1;
+        const int zz = pcz + (s / 3) - 1;
+        int cnt = 0;
+        int a = INT_MAX;
+        if (t < pairs && xs + k <= xe && yy >= 0 && yy < pg.dims.y && zz >= 0 &&
+            zz < pg.dims.z) {
+            const uint32_t ck = pg.cellOff + key_of(xs + k, yy, zz, pg.dims);
+            const int cs = g.cellStart[ck];
+            if (cs != -1) {
+                cnt = g.cellEnd[ck] - cs;
+                a = cs;
+            }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int v = __shfl_up(incl, o, 16);
+            if (k >= o) incl += v;
+        }
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) a = min(a, __shfl_xor(a, o, 16));
+        if (t < pairs) {
+            L.off[p][s][k + 1] = incl;
+            if (k == 0) {
+                L.off[p][s][0] = 0;
+                L.first[p][s] = a == INT_MAX ? 0 : a;
+            }
+        }
     }
-    if (starts) {
-        const int rank = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-            static_cast<uint32_t>(ballot >> 32),
-            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ballot), 0u)));
-        m.tiles[before + rank] = static_cast<int>(t);
-        m.tileFirst[before + rank] = static_cast<int>(i);
+    wave_sync();
+
+    // The lane's piece, its segment cells and its cell among them.
+    const int piece = lane >= split ? 1 : 0;
+    const int xs = max((piece != 0 ? xa1 : xa0) - 1, 0);
+    const int ncell = min((piece != 0 ? xb1 : xb0) + 1, gm.dims.x - 1) - xs + 1;
+    const int mc = cx - xs;
+    const int loCell = max(mc - 1, 0);
+    const int hiCell = min(mc + 1, ncell - 1) + 1;
+    const float3 ip = test_point<kSkin>(gm.box, g.radius, ld4(g.p, gc));
+    const pbf_container box0 = kSkin ? geom_of<kBatched>(g, w0).box : pbf_container{};
+    const pbf_container box1 =
+        kSkin ? geom_of<kBatched>(g, w0 + static_cast<size_t>(first1)).box : pbf_container{};
+
+    for (int s = 0; s < kSegs; ++s) {
+        const int len0 = L.off[0][s][kSegCells];
+        const int len1 = pieces > 1 ? L.off[1][s][kSegCells] : 0;
+        const int total = len0 + len1;
+        if (total == 0) continue;
+        const bool staged = total <= kStage;
+        if (staged) {
+            // Segment 0 then segment 1, kStageUnroll loads in flight per lane.
+            const long long f0 = L.first[0][s];
+            const long long f1 = pieces > 1 ? static_cast<long long>(L.first[1][s]) - len0 : 0;
+            for (int q0 = lane; q0 < total; q0 += 64 * kStageUnroll) {
+                float4 v[kStageUnroll];
+#pragma unroll
+                for (int u = 0; u < kStageUnroll; ++u) {
+                    const int q = q0 + 64 * u;
+                    if (q < total) v[u] = ld4(g.p, static_cast<size_t>((q < len0 ? f0 : f1) + q));
+                }
+#pragma unroll
+                for (int u = 0; u < kStageUnroll; ++u) {
+                    const int q = q0 + 64 * u;
+                    if (q < total) {
+                        const float3 r = test_point<kSkin>(q < len0 ? box0 : box1, g.radius, v[u]);
+                        float* o = L.xyz + 3 * q;
+                        o[0] = r.x;
+                        o[1] = r.y;
+                        o[2] = r.z;
+                    }
+                }
+            }
+            wave_sync();
+        }
+        if (active) {
+            const int lo = L.off[piece][s][loCell];
+            const int hi = L.off[piece][s][hiCell];
+            if (hi > lo) {
+                const size_t first = static_cast<size_t>(L.first[piece][s]);
+                const int selfq = s == 4 ? static_cast<int>(gi - first) : -1;
+                if (staged) {
+                    const LdsRow src{L.xyz, piece != 0 ? len0 : 0};
+                    sweep_row(g, src, ip, lo, hi, selfq, first, st, L, lane);
+                } else {
+                    const GlobalRow<kSkin> src{g.p, first, (piece != 0 ? len1 : len0) - 1, gm.box,
+                                               g.radius};
+                    sweep_row(g, src, ip, lo, hi, selfq, first, st, L, lane);
+                }
+            }
+        }
+        wave_sync();  // the row's readers are done before the next row is staged
     }
-    lds_barrier();
-    if (total <= kMaxTilePasses) {
-        for (int k = 0; k < total; ++k)
-            sweep_tile<kSkin>(g, geom_of<kBatched>(g, static_cast<size_t>(m.tileFirst[k])), m,
-                              lds, m.tiles[k]);
-    } else {
-        // The particles of every tile that starts here, one lane each.
-        const size_t first = static_cast<size_t>(m.tileFirst[0]);
-        const size_t end = g.tileEnd[m.tiles[total - 1]];
-        for (size_t q = first + static_cast<size_t>(tid); q < end; q += kThreads)
-            sweep_lane<kBatched, kSkin>(g, q);
-    }
+    if (active) finish_particle(g, gi, st, L, lane);
 }
 
 }  // namespace
@@ -515,7 +484,7 @@ k_build_lists(Grid g) {
 namespace pbfint {
 
 size_t sweep_tile_count(pbf_int3 dims) {
-    const size_t chunks = (static_cast<size_t>(dims.x) + kTileW - 1) / kTileW;
+    const size_t chunks = (static_cast<size_t>(dims.x) + kSweepTileW - 1) / kSweepTileW;
     return chunks * static_cast<size_t>(dims.y) * static_cast<size_t>(dims.z);
 }
 
@@ -528,19 +497,19 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "grouped lists need a multiple of 4 slots");
     if (reinterpret_cast<uintptr_t>(nbr) % alignof(uint4) != 0)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "grouped lists need 16-byte alignment");
+    if (sg.n > 0xffffffffull)
+        return pbf_fail(PBF_ERR_LENGTH, "too many particles for 32-bit list entries");
     Grid g;
     g.p = p;
     g.cellStart = sg.cellStart;
     g.cellEnd = sg.cellEnd;
     g.sortedKeys = sg.sortedKeys;
-    g.tileEnd = sg.tileEnd;
     g.nbr = reinterpret_cast<uint4*>(nbr);
     g.cnt = cnt;
     g.status = status;
     g.n = sg.n;
     g.stride = stride;
     g.dims = sg.dims;
-    g.chunks = (sg.dims.x + kTileW - 1) / kTileW;
     g.h2 = sg.h * sg.h;
     g.maxNeighbors = sg.maxNeighbors;
     g.st = sg.st;
@@ -550,9 +519,6 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
     g.fullBit = sg.skin ? PBF_STATUS_LIST_STALE : PBF_STATUS_NEIGHBOR_OVERFLOW;
     if (sg.skin && sg.st.geom == nullptr && sg.box == nullptr)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "skin lists need the container");
-    const size_t tiles = sg.st.geom != nullptr ? 0 : sweep_tile_count(sg.dims);
-    if (tiles > 0x7fffffffull)
-        return pbf_fail(PBF_ERR_LENGTH, "grid has too many rows for the neighbour sweep");
     const size_t tasks = (sg.n + kThreads - 1) / kThreads;
     if (tasks > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "too many particles for the neighbour sweep");
