@@ -218,29 +218,53 @@ HostArray host_array_kind(const void* p) {
 // by the Delta-p pass itself), and the post-solve phase's records.  Sized
 // for the solver's iteration count.
 struct SubstepTape {
-    std::vector<DeviceBuffer<std::uint32_t>> ids, nbr;
-    std::vector<DeviceBuffer<int>> cnt;
-    std::vector<DeviceBuffer<float4>> pk, pre;
-    DeviceBuffer<float4> xs, pf, v3, omega, vin;
+    // Every array below lives in one allocation (one device allocation per
+    // substep of a segment rather than one per array).
+    DeviceBuffer<float4> arena;
+    std::size_t n = 0;
+    std::size_t listWords = 0;
+    std::vector<std::uint32_t*> ids, nbr;
+    std::vector<int*> cnt;
+    std::vector<float4*> pk, pre;
+    float4 *xs = nullptr, *pf = nullptr, *v3 = nullptr, *omega = nullptr, *vin = nullptr;
 
-    void allocate(std::size_t n, std::size_t listWords, int iterations) {
+    void allocate(std::size_t n_, std::size_t listWords_, int iterations) {
         const std::size_t builds = static_cast<std::size_t>(iterations) + 1;
-        if (xs.size() >= n && nbr.size() == builds && nbr[0].size() >= listWords) return;
-        ids.resize(builds);
-        nbr.resize(builds);
-        cnt.resize(builds);
-        pk.resize(builds - 1);
-        pre.resize(builds - 1);
+        if (arena.size() > 0 && n >= n_ && nbr.size() == builds && listWords >= listWords_)
+            return;
+        // Sizes in 16-byte units, every array starting on a 16-byte boundary.
+        const auto units = [](std::size_t bytes) { return (bytes + 15) / 16; };
+        const std::size_t u32 = units(n_ * sizeof(std::uint32_t));
+        const std::size_t lists = units(listWords_ * sizeof(std::uint32_t));
+        const std::size_t total = builds * (2 * u32 + lists) + 2 * (builds - 1) * n_ + 5 * n_;
+        arena.allocate(total);
+        float4* at = arena.data();
+        const auto take = [&](std::size_t count) {
+            float4* p = at;
+            at += count;
+            return p;
+        };
+        ids.assign(builds, nullptr);
+        nbr.assign(builds, nullptr);
+        cnt.assign(builds, nullptr);
+        pk.assign(builds - 1, nullptr);
+        pre.assign(builds - 1, nullptr);
         for (std::size_t b = 0; b < builds; ++b) {
-            ids[b].allocate(n);
-            nbr[b].allocate(listWords);
-            cnt[b].allocate(n);
+            ids[b] = reinterpret_cast<std::uint32_t*>(take(u32));
+            nbr[b] = reinterpret_cast<std::uint32_t*>(take(lists));
+            cnt[b] = reinterpret_cast<int*>(take(u32));
         }
         for (std::size_t k = 0; k + 1 < builds; ++k) {
-            pk[k].allocate(n);
-            pre[k].allocate(n);
+            pk[k] = take(n_);
+            pre[k] = take(n_);
         }
-        for (DeviceBuffer<float4>* b : {&xs, &pf, &v3, &omega, &vin}) b->allocate(n);
+        xs = take(n_);
+        pf = take(n_);
+        v3 = take(n_);
+        omega = take(n_);
+        vin = take(n_);
+        n = n_;
+        listWords = listWords_;
     }
 };
 
@@ -373,10 +397,8 @@ struct PBFSolver::State {
         if (tp == nullptr) return;
         if (b < 0 || static_cast<std::size_t>(b) >= tp->ids.size())
             throw std::logic_error("Adjoint tape holds fewer builds than the substep runs");
-        tape_copy(tp->ids[b], id, count);
-        launch_check(pbfint::tape_lists(nbr.data(), tp->nbr[b].data(), cnt.data(), count, stride,
-                                        kMaxNeighbors / 4, stream));
-        tape_copy(tp->cnt[b], cnt, count);
+        launch_check(pbfint::tape_build(nbr.data(), tp->nbr[b], cnt.data(), tp->cnt[b], id.data(),
+                                        tp->ids[b], count, stride, kMaxNeighbors / 4, stream));
     }
 
     ~State() {
@@ -701,12 +723,12 @@ struct PBFSolver::State {
                 tape_build(it);
                 {
                     Scope sc(prof, "lambda", stream);
+                    pbfint::ListPass ll = lpass();
+                    if (SubstepTape* tp = tape()) ll.tapeOut = tp->pk[it];
                     launch_check(pbfint::launch_lambda_guarded(
                         p.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h, P.particleMass,
-                        P.restDensity, P.lambdaEpsilon, stride, lpass(), status.data(),
-                        stream));
+                        P.restDensity, P.lambdaEpsilon, stride, ll, status.data(), stream));
                 }
-                if (SubstepTape* tp = tape()) tape_copy(tp->pk[it], p, n);
                 {
                     Scope sc(prof, "delta_apply_collide", stream);
                     pbfint::ListPass dl = lpass();
@@ -717,7 +739,7 @@ struct PBFSolver::State {
                     dl.clearCells = grid.cellStart.data();
                     dl.clearCount = grid.numCells;
                     dl.xperm = recordsLag ? perm.data() : nullptr;
-                    if (SubstepTape* tp = tape()) dl.prePos = tp->pre[it].data();
+                    if (SubstepTape* tp = tape()) dl.prePos = tp->pre[it];
                     launch_check(pbfint::launch_delta_apply_collide(
                         p.data(), pa.data(), nbr.data(), cnt.data(), kMaxNeighbors, x.data(),
                         n, h, P.particleMass, P.restDensity, P.scorrK, P.scorrN, P.scorrDeltaQ,
@@ -729,35 +751,38 @@ struct PBFSolver::State {
                 build_and_reorder();
                 build_lists();
                 tape_build(P.solverIterations);
-                if (SubstepTape* tp = tape()) {
-                    tape_copy(tp->xs, x, n);
-                    tape_copy(tp->pf, p, n);
-                    tape_copy(tp->vin, vin, n);
-                }
+                if (SubstepTape* tp = tape())
+                    launch_check(pbfint::tape_records(x.data(), tp->xs, p.data(), tp->pf,
+                                                      vin.data(), tp->vin, n, stream));
             }
             if (P.xsphViscosity != 0.0f) {
                 Scope sc(prof, "update_xsph", stream);
+                pbfint::ListPass xl = lpass();
+                if (SubstepTape* tp = tape()) xl.tapeOut = tp->v3;
                 launch_check(pbfint::launch_update_xsph(
                     x.data(), p.data(), v.data(), va.data(), nbr.data(), cnt.data(), kMaxNeighbors,
-                    n, h, P.xsphViscosity, invDt, stride, lpass(), status.data(), stream));
+                    n, h, P.xsphViscosity, invDt, stride, xl, status.data(), stream));
                 std::swap(v, va);
             } else {
                 Scope sc(prof, "update_velocity", stream);
                 launch_check(pbfint::launch_update_velocity_only(x.data(), p.data(), v.data(), n,
                                                                  invDt, status.data(), stream));
+                if (SubstepTape* tp = tape())
+                    hip_check(hipMemcpyAsync(tp->v3, v.data(), n * sizeof(float4),
+                                             hipMemcpyDeviceToDevice, stream));
             }
-            if (SubstepTape* tp = tape()) tape_copy(tp->v3, v, n);
             std::swap(x, p);
             if (P.vorticityStrength != 0.0f) {
                 {
                     Scope sc(prof, "vorticity", stream);
                     // p holds the previous x after the swap and is free until
                     // the next substep's prediction: it takes the records.
+                    pbfint::ListPass vl = lpass();
+                    if (SubstepTape* tp = tape()) vl.tapeOut = tp->omega;
                     launch_check(pbfint::launch_vorticity(
                         x.data(), v.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h,
-                        omega.data(), p.data(), stride, lpass(), status.data(), stream));
+                        omega.data(), p.data(), stride, vl, status.data(), stream));
                 }
-                if (SubstepTape* tp = tape()) tape_copy(tp->omega, omega, n);
                 {
                     Scope sc(prof, "confine_resolve", stream);
                     launch_check(pbfint::launch_confine_resolve(
@@ -772,10 +797,11 @@ struct PBFSolver::State {
                     // Confinement is off, but its strength's derivative at 0
                     // needs the vorticity: computed for the tape only (the
                     // pass changes no state; p takes its records).
+                    pbfint::ListPass vl = lpass();
+                    vl.tapeOut = tp->omega;
                     launch_check(pbfint::launch_vorticity(
                         x.data(), v.data(), nbr.data(), cnt.data(), kMaxNeighbors, n, h,
-                        omega.data(), p.data(), stride, lpass(), status.data(), stream));
-                    tape_copy(tp->omega, omega, n);
+                        omega.data(), p.data(), stride, vl, status.data(), stream));
                 }
                 Scope sc(prof, "resolve_velocities", stream);
                 launch_check(pbfint::launch_resolve_guarded(
@@ -1666,7 +1692,7 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
     ap.scorrDq = P.scorrDeltaQ;
     ap.col = s.collision.state().view();
     const auto build = [&](SubstepTape& tp, int b) {
-        return pbfint::AdjBuild{tp.nbr[b].data(), tp.cnt[b].data(), tp.ids[b].data(), s.stride};
+        return pbfint::AdjBuild{tp.nbr[b], tp.cnt[b], tp.ids[b], s.stride};
     };
 
     const std::size_t segments = m.checkpoints.size();
@@ -1700,11 +1726,11 @@ void PBFAdjoint::backward(const float4* xBar, const float4* vBar, std::size_t pa
             pbfint::AdjBuffers w{m.pb.data(),   m.vb.data(),    m.x0b.data(),  m.v1b.data(),
                                  m.v3b.data(),  m.etab.data(),  m.wb.data(),   m.prb.data(),
                                  m.gsum.data(), m.cterm.data(), m.lamb.data(), m.acc.data()};
-            const pbfint::AdjPost post{build(tp, P.solverIterations), tp.xs.data(), tp.pf.data(), tp.v3.data(),
-                                       tp.omega.data(), tp.vin.data()};
+            const pbfint::AdjPost post{build(tp, P.solverIterations), tp.xs, tp.pf, tp.v3,
+                                       tp.omega, tp.vin};
             launch_check(pbfint::adjoint_post(post, w, n, ap, s.stream));
             for (int it = P.solverIterations; it-- > 0;) {
-                const pbfint::AdjIter iter{build(tp, it), tp.pk[it].data(), tp.pre[it].data()};
+                const pbfint::AdjIter iter{build(tp, it), tp.pk[it], tp.pre[it]};
                 launch_check(pbfint::adjoint_iteration(iter, w, n, ap, s.stream));
             }
             launch_check(pbfint::adjoint_predict(w, n, ap.dt, s.stream));

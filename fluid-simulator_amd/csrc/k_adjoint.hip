@@ -92,6 +92,12 @@ struct Nbr2 {
     uint32_t j;
     float4 a, b;
 };
+// Three per-slot records of a neighbour, loaded together with the group's
+// other entries' (for_each_neighbour keeps a group's loads in flight).
+struct Nbr3 {
+    uint32_t j;
+    float4 a, b, c;
+};
 
 // A 3x3 matrix by rows.
 struct M3 {
@@ -364,10 +370,10 @@ __global__ void __launch_bounds__(kBlock)
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
     for_each_neighbour<true>(
         a.b.nbr, a.b.stride, i, a.b.cnt[i],
-        [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.om, j)}; },
-        [&](const Nbr2& r) {
+        [&](uint32_t j) { return Nbr3{j, ld4(a.pf, j), ld4(a.om, j), ld4(etab, j)}; },
+        [&](const Nbr3& r) {
             const float3 d = pi - f3(r.a);
-            const float3 eb = ebi + f3(ld4(etab, r.j));
+            const float3 eb = ebi + f3(r.c);
             wl -= dot3(eb, spiky(d, s));
             pacc = pacc + (r.b.w - wi.w) * spiky_jac(d, eb, s);
         });
@@ -397,10 +403,10 @@ __global__ void __launch_bounds__(kBlock)
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
     for_each_neighbour<true>(
         a.b.nbr, a.b.stride, i, a.b.cnt[i],
-        [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.v3, j)}; },
-        [&](const Nbr2& r) {
+        [&](uint32_t j) { return Nbr3{j, ld4(a.pf, j), ld4(a.v3, j), ld4(wb, j)}; },
+        [&](const Nbr3& r) {
             const float3 d = pi - f3(r.a);
-            const float3 wbs = wbi + f3(ld4(wb, r.j));
+            const float3 wbs = wbi + f3(r.c);
             vacc = vacc - cross3(wbs, spiky(d, s));
             pacc = pacc + spiky_jac(d, cross3(f3(r.b) - vi, wbs), s);
         });
@@ -432,13 +438,13 @@ __global__ void __launch_bounds__(kBlock)
         float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
         for_each_neighbour<true>(
             a.b.nbr, a.b.stride, i, a.b.cnt[i],
-            [&](uint32_t j) { return Nbr2{j, ld4(a.pf, j), ld4(a.xs, j)}; },
-            [&](const Nbr2& r) {
+            [&](uint32_t j) { return Nbr3{j, ld4(a.pf, j), ld4(a.xs, j), ld4(v3b, j)}; },
+            [&](const Nbr3& r) {
                 const float3 pj = f3(r.a);
                 const float3 d = pi - pj;
                 const float W = poly6_d2(dot3(d, d), s);
                 const float3 dv = inv_dt * (pj - f3(r.b)) - v2i;
-                const float3 vbj = f3(ld4(v3b, r.j));
+                const float3 vbj = f3(r.c);
                 v2b = v2b + (a.visc * W) * (vbj - vbi);
                 cpart += W * dot3(vbi, dv);
                 pacc = pacc + (a.visc * dot3(dv, vbi - vbj)) * poly6_grad(d, s);
@@ -566,9 +572,11 @@ __global__ void __launch_bounds__(kBlock)
     float3 pacc = make_float3(0.0f, 0.0f, 0.0f);
     for_each_neighbour<true>(
         a.b.nbr, a.b.stride, i, a.b.cnt[i],
-        [&](uint32_t j) { return Nbr2{j, ld4(a.pk, j), ld4(gsum, j)}; },
-        [&](const Nbr2& r) {
-            const float2 bj = lambda_bars(lamb[r.j], cterm[r.j], r.b.w, a.eps);
+        [&](uint32_t j) {
+            return Nbr3{j, ld4(a.pk, j), ld4(gsum, j), make_float4(lamb[j], cterm[j], 0.0f, 0.0f)};
+        },
+        [&](const Nbr3& r) {
+            const float2 bj = lambda_bars(r.c.x, r.c.y, r.b.w, a.eps);
             const float3 d = pi - f3(r.a);
             const float3 g = (-gs) * spiky(d, s);
             const float3 gbi = (2.0f * bi.y) * (g + gsv);
@@ -603,26 +611,54 @@ __global__ void __launch_bounds__(kBlock)
     st4(v1b, i, v);
 }
 
-// The groups of a grouped list that hold entries: ceil(cnt / 4) of each
-// slot (the list kernels read no others), at most maxGroups.
+// One grid build into the tape: the list groups that hold entries, the
+// counts and the slot-to-id map (tape_build).
 __global__ void __launch_bounds__(kBlock)
-    k_tape_lists(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                 const int* __restrict__ cnt, size_t n, size_t stride, int maxGroups) {
+    k_tape_build(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                 const int* __restrict__ cnt, int* __restrict__ cntOut,
+                 const uint32_t* __restrict__ ids, uint32_t* __restrict__ idsOut, size_t n,
+                 size_t stride, int maxGroups) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int groups = min((cnt[i] + 3) / 4, maxGroups);
+    const int c = cnt[i];
+    cntOut[i] = c;
+    idsOut[i] = ids[i];
+    const int groups = min((c + 3) / 4, maxGroups);
     for (int g = 0; g < groups; ++g) dst[static_cast<size_t>(g) * stride + i] = src[static_cast<size_t>(g) * stride + i];
+}
+
+// Three record arrays into the tape (tape_records).
+__global__ void __launch_bounds__(kBlock)
+    k_tape_records(const float4* __restrict__ a, float4* __restrict__ aOut,
+                   const float4* __restrict__ b, float4* __restrict__ bOut,
+                   const float4* __restrict__ c, float4* __restrict__ cOut, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aOut[i] = a[i];
+    bOut[i] = b[i];
+    cOut[i] = c[i];
 }
 
 }  // namespace
 
 namespace pbfint {
 
-pbf_status tape_lists(const uint32_t* src, uint32_t* dst, const int* cnt, size_t n, size_t stride,
+pbf_status tape_build(const uint32_t* lists, uint32_t* listsOut, const int* cnt, int* cntOut,
+                      const uint32_t* ids, uint32_t* idsOut, size_t n, size_t stride,
                       int maxGroups, void* stream) {
     if (n == 0) return PBF_OK;
-    PBF_LAUNCH(k_tape_lists, grid_for(n), kBlock, stream, reinterpret_cast<const uint4*>(src),
-               reinterpret_cast<uint4*>(dst), cnt, n, stride, maxGroups);
+    PBF_LAUNCH(k_tape_build, grid_for(n), kBlock, stream, reinterpret_cast<const uint4*>(lists),
+               reinterpret_cast<uint4*>(listsOut), cnt, cntOut, ids, idsOut, n, stride, maxGroups);
+}
+
+pbf_status tape_records(const pbf_float4* a, pbf_float4* aOut, const pbf_float4* b,
+                        pbf_float4* bOut, const pbf_float4* c, pbf_float4* cOut, size_t n,
+                        void* stream) {
+    if (n == 0) return PBF_OK;
+    PBF_LAUNCH(k_tape_records, grid_for(n), kBlock, stream, reinterpret_cast<const float4*>(a),
+               reinterpret_cast<float4*>(aOut), reinterpret_cast<const float4*>(b),
+               reinterpret_cast<float4*>(bOut), reinterpret_cast<const float4*>(c),
+               reinterpret_cast<float4*>(cOut), n);
 }
 
 pbf_status adjoint_post(const AdjPost& t, const AdjBuffers& w, size_t n, const AdjParams& prm,

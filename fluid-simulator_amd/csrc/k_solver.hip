@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(kBlock)
     k_lambda_list(const pbf_float4* p, const uint32_t* __restrict__ nbr,
                   const int* __restrict__ cnt, int maxN, size_t n, float h, float mass, float rho0,
                   float eps, float* lambda, size_t lstride, size_t stride,
-                  uint32_t* __restrict__ status, int exactMax, int ppt) {
+                  uint32_t* __restrict__ status, int exactMax, int ppt,
+                  pbf_float4* __restrict__ tapeOut) {
     for_each_particle<kFast>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (status != nullptr && *status != 0u) return;
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(kBlock)
     const int c = cnt[i];
     if (!valid_count(c, maxN)) {
         lambda[i * lstride] = qnan();
+        if (tapeOut != nullptr) st4(tapeOut, i, make_float4(pi.x, pi.y, pi.z, qnan()));
         return;
     }
     const Sph s = sph_consts(h);
@@ -156,7 +158,9 @@ __global__ void __launch_bounds__(kBlock)
         });
     sg += gix * gix + giy * giy + giz * giz;
     const double C = rho / static_cast<double>(rho0) - 1.0;
-    lambda[i * lstride] = static_cast<float>(-C / (sg + static_cast<double>(eps)));
+    const float lam = static_cast<float>(-C / (sg + static_cast<double>(eps)));
+    lambda[i * lstride] = lam;
+    if (tapeOut != nullptr) st4(tapeOut, i, make_float4(pi.x, pi.y, pi.z, lam));
     if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
     });
 }
@@ -268,13 +272,14 @@ __global__ void __launch_bounds__(kBlock)
                      const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                      size_t n, float h, pbf_float4* __restrict__ omega,
                      pbf_float4* __restrict__ pw, size_t stride, uint32_t* __restrict__ status,
-                     int exactMax, int ppt) {
+                     int exactMax, int ppt, pbf_float4* __restrict__ tapeOut) {
     for_each_particle<kFast>(ppt, [&](const size_t i) {
     if (i >= n) return;
     const int c = cnt[i];
     const float4 pi = ld4(p, i);
     if (!valid_count(c, maxN)) {
         st4(omega, i, make_float4(qnan(), qnan(), qnan(), 0.0f));
+        if (tapeOut != nullptr) st4(tapeOut, i, make_float4(qnan(), qnan(), qnan(), 0.0f));
         if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, qnan()));
         return;
     }
@@ -297,6 +302,7 @@ __global__ void __launch_bounds__(kBlock)
     if (exactMax > 0 && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
     const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
+    if (tapeOut != nullptr) st4(tapeOut, i, make_float4(ox, oy, oz, wl));
     if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, wl));
     });
 }
@@ -631,6 +637,14 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
+// (p* - x) / dt as the reference's velocity update stores it
+// (pbf_solver_kernels.cu:207): rounded, never fused into the XSPH sums that
+// read it back.
+__device__ __forceinline__ float stored_velocity(float pc, float xc, float inv_dt) {
+#pragma clang fp contract(off)
+    return (pc - xc) * inv_dt;
+}
+
 // updateVelocityAndPosition + applyXsphViscosity (pbf_solver.cu:284-298).
 // Velocities v = (p* - x) / dt are rebuilt on the fly for i and every
 // neighbour with the same FP32 expression the reference stores, so the
@@ -644,19 +658,21 @@ __global__ void __launch_bounds__(kBlock)
                   const pbf_float4* __restrict__ vold, pbf_float4* __restrict__ vout,
                   const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int maxN,
                   size_t n, float h, float visc, float inv_dt, size_t stride,
-                  uint32_t* __restrict__ status, int exactMax, int ppt) {
+                  uint32_t* __restrict__ status, int exactMax, int ppt,
+                  pbf_float4* __restrict__ tapeOut) {
     for_each_particle<true>(ppt, [&](const size_t i) {
     if (i >= n) return;
     if (*status != 0u) return;
     const float4 xi = ld4(x, i);
     const float4 pi = ld4(p, i);
     const float vw = vold[i].w;
-    const float vix = (pi.x - xi.x) * inv_dt;
-    const float viy = (pi.y - xi.y) * inv_dt;
-    const float viz = (pi.z - xi.z) * inv_dt;
+    const float vix = stored_velocity(pi.x, xi.x, inv_dt);
+    const float viy = stored_velocity(pi.y, xi.y, inv_dt);
+    const float viz = stored_velocity(pi.z, xi.z, inv_dt);
     const int c = cnt[i];
     if (!valid_count(c, maxN)) {
         st4(vout, i, make_float4(qnan(), qnan(), qnan(), vw));
+        if (tapeOut != nullptr) st4(tapeOut, i, make_float4(qnan(), qnan(), qnan(), vw));
         return;
     }
     const Sph s = sph_consts(h);
@@ -666,9 +682,9 @@ __global__ void __launch_bounds__(kBlock)
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &xj = r.b;
-            const float vjx = (pj.x - xj.x) * inv_dt;
-            const float vjy = (pj.y - xj.y) * inv_dt;
-            const float vjz = (pj.z - xj.z) * inv_dt;
+            const float vjx = stored_velocity(pj.x, xj.x, inv_dt);
+            const float vjy = stored_velocity(pj.y, xj.y, inv_dt);
+            const float vjz = stored_velocity(pj.z, xj.z, inv_dt);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             float d2;
             if (kSkin) {
@@ -682,7 +698,9 @@ __global__ void __launch_bounds__(kBlock)
             cy += (vjy - viy) * w;
             cz += (vjz - viz) * w;
         });
-    st4(vout, i, make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw));
+    const float4 vnew = make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw);
+    st4(vout, i, vnew);
+    if (tapeOut != nullptr) st4(tapeOut, i, vnew);
     if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
     });
 }
@@ -784,7 +802,8 @@ pbf_status pbf_k_compute_lambda(const pbf_float4* p, const uint32_t* nbr, const 
                                 float* lambda, size_t stride, void* stream) {
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_lambda_list<false>, grid_for(n), kBlock, stream, p, nbr, cnt, maxN, n, h, mass,
-               rho0, eps, lambda, size_t{1}, stride, static_cast<uint32_t*>(nullptr), 0, 1);
+               rho0, eps, lambda, size_t{1}, stride, static_cast<uint32_t*>(nullptr), 0, 1,
+               static_cast<pbf_float4*>(nullptr));
 }
 
 pbf_status pbf_k_compute_delta_position(const pbf_float4* p, const uint32_t* nbr, const int* cnt,
@@ -821,7 +840,7 @@ pbf_status pbf_k_compute_vorticity(const pbf_float4* p, const pbf_float4* v, con
     if (n == 0) return PBF_OK;
     PBF_LAUNCH(k_vorticity_list<false>, grid_for(n), kBlock, stream, p, v, nbr, cnt, maxN, n, h,
                omega, static_cast<pbf_float4*>(nullptr), stride, static_cast<uint32_t*>(nullptr),
-               0, 1);
+               0, 1, static_cast<pbf_float4*>(nullptr));
 }
 
 pbf_status pbf_k_apply_vorticity_confinement(const pbf_float4* p, const uint32_t* nbr,
@@ -961,9 +980,10 @@ pbf_status launch_lambda_guarded(pbf_float4* p, const uint32_t* nbr, const int* 
     float* lambda = reinterpret_cast<float*>(p) + 3;  // the .w words
     if (lp.p0 != nullptr)
         PBF_LAUNCH((k_lambda_list<true, true>), grid_for(n, lp.block * lp.ppt), lp.block, stream, p, nbr, cnt, maxN, n,
-                   h, mass, rho0, eps, lambda, size_t{4}, stride, status, lp.exactMax, lp.ppt);
+                   h, mass, rho0, eps, lambda, size_t{4}, stride, status, lp.exactMax, lp.ppt,
+                   lp.tapeOut);
     PBF_LAUNCH((k_lambda_list<true, false>), grid_for(n, lp.block * lp.ppt), lp.block, stream, p, nbr, cnt, maxN, n, h,
-               mass, rho0, eps, lambda, size_t{4}, stride, status, 0, lp.ppt);
+               mass, rho0, eps, lambda, size_t{4}, stride, status, 0, lp.ppt, lp.tapeOut);
 }
 
 pbf_status launch_delta_apply_collide(const pbf_float4* pin, pbf_float4* pout, const uint32_t* nbr,
@@ -992,9 +1012,9 @@ pbf_status launch_update_xsph(const pbf_float4* x, const pbf_float4* p, const pb
                               const ListPass& lp, uint32_t* status, void* stream) {
     if (lp.exactMax > 0)
         PBF_LAUNCH(k_update_xsph<true>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, p, vold, vout, nbr, cnt,
-                   maxN, n, h, visc, inv_dt, stride, status, lp.exactMax, lp.ppt);
+                   maxN, n, h, visc, inv_dt, stride, status, lp.exactMax, lp.ppt, lp.tapeOut);
     PBF_LAUNCH(k_update_xsph<false>, grid_for(n, lp.block * lp.ppt), lp.block, stream, x, p, vold, vout, nbr, cnt, maxN,
-               n, h, visc, inv_dt, stride, status, 0, lp.ppt);
+               n, h, visc, inv_dt, stride, status, 0, lp.ppt, lp.tapeOut);
 }
 
 pbf_status launch_update_velocity_only(const pbf_float4* x, const pbf_float4* p, pbf_float4* v,
@@ -1008,7 +1028,7 @@ pbf_status launch_vorticity(const pbf_float4* p, const pbf_float4* v, const uint
                             pbf_float4* pw, size_t stride, const ListPass& lp, uint32_t* status,
                             void* stream) {
     PBF_LAUNCH(k_vorticity_list<true>, grid_for(n, lp.block * lp.ppt), lp.block, stream, p, v, nbr, cnt, maxN, n, h,
-               omega, pw, stride, status, lp.exactMax, lp.ppt);
+               omega, pw, stride, status, lp.exactMax, lp.ppt, lp.tapeOut);
 }
 
 pbf_status launch_confine_resolve(const pbf_float4* x, const pbf_float4* pw, const uint32_t* nbr,

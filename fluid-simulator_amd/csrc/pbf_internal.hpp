@@ -64,6 +64,10 @@ struct ListPass {
     // Delta-p: with prePos set, also the positions after Delta-p and before
     // the collision solve (the adjoint's tape).
     pbf_float4* prePos = nullptr;
+    // Lambda, XSPH and vorticity: with tapeOut set, also their output record
+    // of every particle ((p*, lambda), the velocity, omega) for the
+    // adjoint's tape.
+    pbf_float4* tapeOut = nullptr;
 };
 
 // Grid (k_grid.hip)
@@ -237,10 +241,16 @@ pbf_status adjoint_iteration(const AdjIter& t, const AdjBuffers& w, size_t n,
                              const AdjParams& p, void* stream);
 // Integrate backward: completes x0b and v1b from pb.
 pbf_status adjoint_predict(const AdjBuffers& w, size_t n, float dt, void* stream);
-// Copies the groups of the grouped lists src (stride slots per group row)
-// that hold entries, ceil(cnt[i] / 4) of slot i (at most maxGroups), to dst.
-pbf_status tape_lists(const uint32_t* src, uint32_t* dst, const int* cnt, size_t n, size_t stride,
+// One grid build into the adjoint's tape: the groups of the grouped lists
+// (stride slots per group row) that hold entries, ceil(cnt[i] / 4) of slot i
+// (at most maxGroups), the counts and the slot-to-id map.
+pbf_status tape_build(const uint32_t* lists, uint32_t* listsOut, const int* cnt, int* cntOut,
+                      const uint32_t* ids, uint32_t* idsOut, size_t n, size_t stride,
                       int maxGroups, void* stream);
+// Three record arrays of n records into the tape, in one launch.
+pbf_status tape_records(const pbf_float4* a, pbf_float4* aOut, const pbf_float4* b,
+                        pbf_float4* bOut, const pbf_float4* c, pbf_float4* cOut, size_t n,
+                        void* stream);
 
 // Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
 // have cell size >= h (a 3x3x3 stencil) and the working set must be in the
