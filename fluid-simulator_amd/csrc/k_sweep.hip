@@ -57,10 +57,10 @@ constexpr int kSegCells = 16;              // cells of a segment: one scan group
 constexpr int kMaxWidth = kSegCells - 2;   // cells a piece may span
 constexpr int kPieces = 2;                 // grid rows a staged wave may span
 constexpr int kSegs = 9;                   // (dz, dy) rows, dz-major
-constexpr int kStage = 192;                // staged records per row (both pieces)
+constexpr int kStage = 160;                // staged records per row (both pieces)
 constexpr int kPad = 4;                    // test reads run up to 3 records past a window
 constexpr int kStageUnroll = 3;            // staging loads in flight per lane
-constexpr int kBufRows = 12;               // per-lane append column (see append)
+constexpr int kBufRows = 8;                // per-lane append column (see append)
 
 struct Grid {
     const pbf_float4* p;
@@ -115,9 +115,9 @@ __device__ __forceinline__ float3 test_point(const pbf_container& box, float rad
 // apart, where 16-byte records would share one bank group.
 struct WaveLds {
     float xyz[3 * (kStage + kPad)];
-    int off[kPieces][kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
-    int first[kPieces][kSegs];               // working-set index of the segment's start
-    uint32_t buf[kBufRows][64];              // row r of lane l at buf[r][l]
+    uint16_t off[kPieces][kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
+    int first[kPieces][kSegs];  // working-set index of the segment's start
+    uint32_t buf[kBufRows][64];  // row r of lane l at buf[r][l]
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
@@ -185,8 +185,8 @@ __device__ __forceinline__ uint32_t push4(uint32_t mask, float h2, float d0, flo
 
 // A particle's list under construction: the next group's address, the
 // neighbours found (may pass maxNeighbors), the entries kept (at most
-// maxNeighbors), and how many of them wait in the open group (rows 4 ..
-// 4 + open - 1 of the lane's append column).
+// maxNeighbors), and how many of them wait in the open group (rows 0 ..
+// open - 1 of the lane's append column).
 struct ListState {
     uint4* out;
     int count;
@@ -198,11 +198,9 @@ struct ListState {
 // b), read from the top down, entry base + lz) to the list.
 //
 // Each pass takes the next four set bits (the entries past the lane's last
-// are garbage and land beyond its open group), writes them to rows
-// 4 + open .. 7 + open of the lane's column, and when the open group (rows
-// 4 .. 7) fills, stores it and writes the four again at rows open .. open+3:
-// the entries past the stored group then sit at rows 4 .. in order, and the
-// rest falls in the spare rows 0 .. 3.
+// are garbage and land beyond its open group) and writes them to rows
+// open .. open + 3 of the lane's column; when the open group (rows 0 .. 3)
+// fills, it is stored and the entries past it move down four rows.
 __device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t base,
                                        ListState& st, WaveLds& L, int lane) {
     int hits = __builtin_popcount(mask);
@@ -222,18 +220,22 @@ __device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t ba
             e[u] = base + static_cast<uint32_t>(lz);
             mask &= 0x7fffffffu >> (lz & 31);
         }
-        uint32_t* col = &L.buf[4 + st.open][lane];
+        uint32_t* col = &L.buf[st.open][lane];
 #pragma unroll
         for (int u = 0; u < 4; ++u) col[64 * u] = e[u];
         const int taken = min(rem, 4);
         st.stored += taken;
         const int open = st.open + taken;
         if (open >= 4) {
-            *st.out = make_uint4(L.buf[4][lane], L.buf[5][lane], L.buf[6][lane], L.buf[7][lane]);
+            *st.out = make_uint4(L.buf[0][lane], L.buf[1][lane], L.buf[2][lane], L.buf[3][lane]);
             st.out += g.stride;
-            uint32_t* low = &L.buf[st.open][lane];
+            // The entries past the stored group move down four rows; the
+            // others go to row 7, free once the group is stored.
 #pragma unroll
-            for (int u = 0; u < 4; ++u) low[64 * u] = e[u];
+            for (int u = 0; u < 4; ++u) {
+                const int r = st.open + u - 4;
+                L.buf[r >= 0 ? r : 7][lane] = e[u];
+            }
             st.open = open - 4;
         } else {
             st.open = open;
@@ -278,9 +280,9 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
 __device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const ListState& st,
                                                 const WaveLds& L, int lane) {
     if (st.open > 0) {
-        const uint32_t a = L.buf[4][lane];
-        const uint32_t b = st.open > 1 ? L.buf[5][lane] : a;
-        const uint32_t c = st.open > 2 ? L.buf[6][lane] : b;
+        const uint32_t a = L.buf[0][lane];
+        const uint32_t b = st.open > 1 ? L.buf[1][lane] : a;
+        const uint32_t c = st.open > 2 ? L.buf[2][lane] : b;
         *st.out = make_uint4(a, b, c, c);
     }
     g.cnt[gi] = st.count;
@@ -322,9 +324,12 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, const Geom& gm, size_t
 }
 
 // One wave per 64 sorted particles (four independent waves per workgroup).
-// Six workgroups fit a CU's LDS (6.6 KB per wave).
+// Eight workgroups fit a CU's LDS (4.6 KB per wave), eight waves per SIMD:
+// more waves hide more of the rows' staging loads (measured: 6, 7 and 8
+// waves per SIMD give 1047, 992 and 966 us per launch on the 8M double dam
+// break).
 template <bool kBatched, bool kSkin>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_build_lists(Grid g) {
     __shared__ WaveLds ldsAll[kWaves];
     const int lane = static_cast<int>(threadIdx.x) & 63;
@@ -371,8 +376,11 @@ k_build_lists(Grid g) {
         return;
     }
 
-    // Segment offsets: sixteen lanes per (piece, row), one cell each.
+    // Segment offsets: sixteen lanes per (piece, row), one cell each.  A
+    // segment of more than 65,535 particles (offsets are 16-bit) sends the
+    // wave to the per-lane sweep.
     const int pairs = pieces * kSegs;
+    bool deep = false;
     for (int t0 = 0; t0 < pairs; t0 += 4) {
         const int t = t0 + (lane >> 4);
         const int k = lane & 15;
@@ -403,6 +411,7 @@ k_build_lists(Grid g) {
             const int v = __shfl_up(incl, o, 16);
             if (k >= o) incl += v;
         }
+        deep |= incl > 0xffff;
 #pragma unroll
         for (int o = 8; o >= 1; o >>= 1) a = min(a, __shfl_xor(a, o, 16));
         if (t < pairs) {
@@ -414,6 +423,13 @@ k_build_lists(Grid g) {
         }
     }
     wave_sync();
+    if (__ballot(deep) != 0ull) {
+        if (active) {
+            sweep_lane<kSkin>(g, gm, gi, key, st, L, lane);
+            finish_particle(g, gi, st, L, lane);
+        }
+        return;
+    }
 
     // The lane's piece, its segment cells and its cell among them.
     const int piece = lane >= split ? 1 : 0;
