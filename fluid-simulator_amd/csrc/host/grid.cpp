@@ -121,8 +121,7 @@ void GridCore::sortAndRanges(std::size_t n, bool resort, hipStream_t stream) {
     sortKeys(n, resort, stream);
     launch_check(pbfint::launch_fill_cells(cellStart.data(), cellEnd.data(), numCells, stream));
     launch_check(pbfint::launch_cell_ranges(sortedKeys.data(), cellStart.data(), cellEnd.data(), n,
-                                            dims, nullptr, pbfint::SceneTable{}, stream,
-                                            splitters.data()));
+                                            stream, splitters.data()));
 }
 
 SpatialGrid::SpatialGrid() : state_(new State()) {}

@@ -290,14 +290,12 @@ struct PBFSolver::State {
     DeviceBuffer<std::uint32_t> nbr;      // grouped neighbour lists of the last build
     DeviceBuffer<int> cnt;
     std::size_t stride = 0;
-    DeviceBuffer<std::uint32_t> tileEnd;  // sweep tile ends of the last build
 
     // Skin lists: skin > 0 builds the grid once per substep, on cells of
     // h * (1 + skin) (sgrid); a frame whose lists went stale is run again
     // from the checkpoint with a build per iteration.
     float skin = 0.0f;
     GridCore sgrid;
-    DeviceBuffer<std::uint32_t> stileEnd;
     DeviceBuffer<pbfint::SceneGeom> sceneGeomSkin;
     DeviceBuffer<float4> ckx, ckv;
     DeviceBuffer<std::uint32_t> ckid;
@@ -319,7 +317,6 @@ struct PBFSolver::State {
                                   sceneContainers.data(), sceneOf.data()};
     }
     GridCore& grid_of(bool sk) { return sk ? sgrid : grid; }
-    std::uint32_t* tiles_of(bool sk) { return sk ? stileEnd.data() : tileEnd.data(); }
     // Workgroup size of the neighbour passes (a tuning: results do not
     // depend on it).
     int passBlock = 256;
@@ -346,7 +343,7 @@ struct PBFSolver::State {
     // (Re)builds the skin grid for the current skin, bounds and scenes.
     void make_skin_grid(std::size_t capacity);
 
-    void allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles);
+    void allocate(std::size_t maxParticles_, std::size_t stride_);
 
     hipStream_t stream = nullptr;
     Profiler prof;
@@ -536,8 +533,7 @@ struct PBFSolver::State {
             }
             launch_check(pbfint::launch_ranges_gather(grid.sortedKeys.data(),
                                                       grid.cellStart.data(), grid.cellEnd.data(),
-                                                      n, grid.dims, tiles_of(sk), scenes(sk), pm,
-                                                      stream, grid.splitters.data()));
+                                                      n, pm, stream, grid.splitters.data()));
         }
         grid.orderedCount = n;  // p* (and what follows it) is in the new order
         std::swap(p, pa);
@@ -556,7 +552,7 @@ struct PBFSolver::State {
     pbfint::SweepGrid sweep_grid(bool sk = false) {
         GridCore& grid = grid_of(sk);
         pbfint::SweepGrid g{grid.cellStart.data(), grid.cellEnd.data(), grid.sortedKeys.data(),
-                            tiles_of(sk), grid.dims, count,
+                            grid.dims, count,
                             sk ? grid.cellSize : params.smoothingRadius, kMaxNeighbors,
                             scenes(sk)};
         if (sk) {
@@ -1020,34 +1016,26 @@ void check_container(float3 minBounds, float3 maxBounds, const SimulationParams&
 }
 
 // The grids of a batch's scenes (their bounds, cells of size cs) laid end
-// to end: global cell and sweep-tile offsets, and slot ranges.
+// to end: global cell offsets and slot ranges.
 std::vector<pbfint::SceneGeom> scene_grids(const std::vector<pbf_container>& boxes,
                                            const std::vector<std::size_t>& first, float cs,
-                                           std::size_t* cellsOut, std::size_t* tilesOut) {
+                                           std::size_t* cellsOut) {
     std::vector<pbfint::SceneGeom> geom(boxes.size());
-    std::size_t cells = 0, tiles = 0;
+    std::size_t cells = 0;
     for (std::size_t k = 0; k < boxes.size(); ++k) {
         std::size_t sceneCells = 0;
         const int3 d = GridCore::dimsFor(boxes[k].min, boxes[k].max, cs, &sceneCells);
-        const std::size_t sceneTiles = pbfint::sweep_tile_count(d);
-        if (cells + sceneCells > std::numeric_limits<std::uint32_t>::max() ||
-            tiles + sceneTiles > std::numeric_limits<std::uint32_t>::max())
+        if (cells + sceneCells > std::numeric_limits<std::uint32_t>::max())
             throw std::length_error("Batched grids exceed 32-bit key capacity");
         pbfint::SceneGeom& g = geom[k];
         g.minBounds = boxes[k].min;
         g.dims = d;
         g.cellOff = static_cast<std::uint32_t>(cells);
-        g.tileOff = static_cast<std::uint32_t>(tiles);
-        g.chunks = static_cast<std::uint32_t>((static_cast<std::size_t>(d.x) + pbfint::kSweepTileW - 1) /
-                                              pbfint::kSweepTileW);
         g.first = static_cast<std::uint32_t>(first[k]);
         g.end = static_cast<std::uint32_t>(first[k + 1]);
-        g.pad = 0;
         cells += sceneCells;
-        tiles += sceneTiles;
     }
     *cellsOut = cells;
-    *tilesOut = tiles;
     return geom;
 }
 
@@ -1058,32 +1046,28 @@ std::vector<pbfint::SceneGeom> scene_grids(const std::vector<pbf_container>& box
 // stencil.  Built whole before it replaces the old one.
 void PBFSolver::State::make_skin_grid(std::size_t capacity) {
     GridCore next;
-    DeviceBuffer<std::uint32_t> nextTiles;
     DeviceBuffer<pbfint::SceneGeom> nextGeom;
     if (skin > 0.0f && capacity > 0) {
         const float cs = params.smoothingRadius * (1.0f + skin);
         if (sceneCount == 0) {
             next = GridCore::create(capacity, grid.minBounds, grid.maxBounds, cs);
-            nextTiles.allocate(pbfint::sweep_tile_count(next.dims));
         } else {
-            std::size_t cells = 0, tiles = 0;
+            std::size_t cells = 0;
             const std::vector<pbfint::SceneGeom> geom =
-                scene_grids(sceneBoxes, sceneFirst, cs, &cells, &tiles);
+                scene_grids(sceneBoxes, sceneFirst, cs, &cells);
             next = GridCore::createKeyed(capacity, cells, cs);
-            nextTiles.allocate(tiles);
             nextGeom.allocate(geom.size());
             nextGeom.copyFromHostToDevice(geom.data(), geom.size());
         }
     }
     drop_graphs();
     sgrid = std::move(next);
-    stileEnd = std::move(nextTiles);
     sceneGeomSkin = std::move(nextGeom);
 }
 
-// Allocates the working set, lists and tile ends for maxParticles particles
-// (after the grid is in place); the solver stays inactive until the end.
-void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_, std::size_t tiles) {
+// Allocates the working set and lists for maxParticles particles (after
+// the grid is in place); the solver stays inactive until the end.
+void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_) {
     for (DeviceBuffer<float4>* b : {&x, &xa, &p, &pa, &pq, &v, &va, &vin, &vina, &omega, &ckx, &ckv})
         b->allocate(maxParticles_);
     id.allocate(maxParticles_);
@@ -1095,7 +1079,6 @@ void PBFSolver::State::allocate(std::size_t maxParticles_, std::size_t stride_, 
     nbr.allocate(stride_ * static_cast<std::size_t>(kMaxNeighbors));
     cnt.allocate(maxParticles_);
     stride = stride_;
-    tileEnd.allocate(tiles);
 }
 
 void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 maxBounds,
@@ -1123,7 +1106,7 @@ void PBFSolver::initialize(std::size_t maxParticles, float3 minBounds, float3 ma
     s.grid = std::move(grid);
     s.collision.setContainer(Container{minBounds, maxBounds});
     s.ensure_stream();
-    s.allocate(maxParticles, stride, pbfint::sweep_tile_count(s.grid.dims));
+    s.allocate(maxParticles, stride);
     s.params = params;
     s.staleFrames = 0;
     s.make_skin_grid(maxParticles);
@@ -1440,8 +1423,8 @@ void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
         boxes[k] = pbf_container{lo, hi};
         first[k + 1] = first[k] + sc.particleCount;
     }
-    std::size_t cells = 0, tiles = 0;
-    const std::vector<pbfint::SceneGeom> geom = scene_grids(boxes, first, h, &cells, &tiles);
+    std::size_t cells = 0;
+    const std::vector<pbfint::SceneGeom> geom = scene_grids(boxes, first, h, &cells);
     std::vector<std::uint32_t> sceneOf(total);
     for (std::size_t k = 0; k < scenes.size(); ++k)
         for (std::size_t i = first[k]; i < first[k + 1]; ++i)
@@ -1456,7 +1439,7 @@ void PBFBatchSolver::initialize(const std::vector<BatchScene>& scenes,
     s.grid = std::move(grid);
     s.collision.setContainer(Container{scenes[0].minBounds, scenes[0].maxBounds});
     s.ensure_stream();
-    s.allocate(total, stride, tiles);
+    s.allocate(total, stride);
     s.sceneGeom.allocate(scenes.size());
     s.sceneGeom.copyFromHostToDevice(geom.data(), geom.size());
     s.sceneContainers.allocate(scenes.size());

@@ -724,58 +724,27 @@ __global__ void __launch_bounds__(kBlock) k_fill_cells(int* __restrict__ cs, int
     }
 }
 
-// Neighbour-sweep tile (kSweepTileW cells of one grid row) of key k at
-// slot i: in a batched step, the tile of the slot's scene, offset into the
-// global tile range.
-template <bool kBatched>
-__device__ __forceinline__ uint32_t sweep_tile_of(uint32_t k, size_t i, uint32_t gx,
-                                                  uint32_t chunks, const pbfint::SceneTable& st) {
-    if (kBatched) {
-        const pbfint::SceneGeom& sg = st.geom[st.sceneOf[i]];
-        const uint32_t lk = k - sg.cellOff;
-        const uint32_t sgx = static_cast<uint32_t>(sg.dims.x);
-        return sg.tileOff + (lk / sgx) * sg.chunks + (lk % sgx) / pbfint::kSweepTileW;
-    }
-    return (k / gx) * chunks + (k % gx) / pbfint::kSweepTileW;
-}
-
-// buildCellRanges (spatial_grid.cu:123-137): half-open [start, end).
-// With tileEnd not null, also records one past the last particle of every
-// non-empty neighbour-sweep tile, which the last particle of the tile
-// writes.  With splitters not null, also the keys at multiples of
-// kResortTile, which the next re-sort splits its merge by.
-template <bool kBatched>
+// buildCellRanges (spatial_grid.cu:123-137): half-open [start, end).  With
+// splitters not null, also the keys at multiples of kResortTile, which the
+// next re-sort splits its merge by.
 __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restrict__ sk,
                                                         int* __restrict__ cs, int* __restrict__ ce,
-                                                        size_t n, uint32_t gx, uint32_t chunks,
-                                                        uint32_t* __restrict__ tileEnd,
-                                                        pbfint::SceneTable st,
-                                                        uint32_t* __restrict__ splitters) {
+                                                        size_t n, uint32_t* __restrict__ splitters) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = sk[i];
     if (splitters != nullptr && i % kResortTile == 0) splitters[i / kResortTile] = k;
     if (i == 0 || sk[i - 1] != k) cs[k] = static_cast<int>(i);
-    const bool lastInCell = i == n - 1 || sk[i + 1] != k;
-    if (!lastInCell) return;
-    ce[k] = static_cast<int>(i + 1);
-    if (tileEnd != nullptr) {
-        const uint32_t t = sweep_tile_of<kBatched>(k, i, gx, chunks, st);
-        if (i == n - 1 || sweep_tile_of<kBatched>(sk[i + 1], i + 1, gx, chunks, st) != t)
-            tileEnd[t] = static_cast<uint32_t>(i + 1);
-    }
+    if (i == n - 1 || sk[i + 1] != k) ce[k] = static_cast<int>(i + 1);
 }
 
 // The cell ranges of the sorted keys (k_cell_ranges) and, in the same
 // pass, the permutation of the working set into that order
 // (gatherSolverState, src/pbf/pbf_solver_kernels.cu:387-407): slot i of the
 // outputs takes the records of slot idx[i].
-template <bool kBatched>
 __global__ void __launch_bounds__(kBlock)
     k_ranges_gather(const uint32_t* __restrict__ sk, int* __restrict__ cs, int* __restrict__ ce,
-                    size_t n, uint32_t gx, uint32_t chunks, uint32_t* __restrict__ tileEnd,
-                    pbfint::SceneTable st, pbfint::Permute pm,
-                    uint32_t* __restrict__ splitters) {
+                    size_t n, pbfint::Permute pm, uint32_t* __restrict__ splitters) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = pm.idx[i];
@@ -792,12 +761,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t k = sk[i];
     if (splitters != nullptr && i % kResortTile == 0) splitters[i / kResortTile] = k;
     if (i == 0 || sk[i - 1] != k) cs[k] = static_cast<int>(i);
-    const bool lastInCell = i == n - 1 || sk[i + 1] != k;
-    if (!lastInCell) return;
-    ce[k] = static_cast<int>(i + 1);
-    const uint32_t t = sweep_tile_of<kBatched>(k, i, gx, chunks, st);
-    if (i == n - 1 || sweep_tile_of<kBatched>(sk[i + 1], i + 1, gx, chunks, st) != t)
-        tileEnd[t] = static_cast<uint32_t>(i + 1);
+    if (i == n - 1 || sk[i + 1] != k) ce[k] = static_cast<int>(i + 1);
 }
 
 // findNeighbors (src/pbf/neighbors.cu:3-91).  Cells are swept z, y, x over
@@ -878,34 +842,20 @@ pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream) {
     PBF_LAUNCH(k_fill_cells, static_cast<unsigned>(blocks), kBlock, stream, cs, ce, cells);
 }
 
-pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                              uint32_t* tileEnd, const SceneTable& st, void* stream,
+pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream,
                               uint32_t* splitters) {
     if (n == 0) return PBF_OK;
-    const uint32_t gx = static_cast<uint32_t>(dims.x);
-    const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
-    if (st.geom != nullptr)
-        PBF_LAUNCH(k_cell_ranges<true>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
-                   tileEnd, st, splitters);
-    PBF_LAUNCH(k_cell_ranges<false>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
-               tileEnd, st, splitters);
+    PBF_LAUNCH(k_cell_ranges, grid_for(n), kBlock, stream, sk, cs, ce, n, splitters);
 }
 
-pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                                uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
+pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, const Permute& pm,
                                 void* stream, uint32_t* splitters) {
     if (n == 0) return PBF_OK;
     if (pm.x == nullptr && pm.permOut == nullptr)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "a p*-only permutation needs permOut");
     if (pm.permOut != nullptr && pm.permOut == pm.permIn)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "permutation composed in place");
-    const uint32_t gx = static_cast<uint32_t>(dims.x);
-    const uint32_t chunks = (gx + kSweepTileW - 1) / kSweepTileW;
-    if (st.geom != nullptr)
-        PBF_LAUNCH(k_ranges_gather<true>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
-                   tileEnd, st, pm, splitters);
-    PBF_LAUNCH(k_ranges_gather<false>, grid_for(n), kBlock, stream, sk, cs, ce, n, gx, chunks,
-               tileEnd, st, pm, splitters);
+    PBF_LAUNCH(k_ranges_gather, grid_for(n), kBlock, stream, sk, cs, ce, n, pm, splitters);
 }
 
 size_t radix_scratch_words(size_t n) {

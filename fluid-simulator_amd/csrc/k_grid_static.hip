@@ -119,6 +119,5 @@ extern "C" pbf_status pbf_grid_build_static(const pbf_float4* d_predicted, size_
     if (st != PBF_OK) return st;
     st = pbfint::launch_fill_cells(d_cellStart, d_cellEnd, cells, stream);
     if (st != PBF_OK) return st;
-    return pbfint::launch_cell_ranges(d_sortedKeys, d_cellStart, d_cellEnd, n, dims, nullptr,
-                                      pbfint::SceneTable{}, stream);
+    return pbfint::launch_cell_ranges(d_sortedKeys, d_cellStart, d_cellEnd, n, stream);
 }

@@ -499,11 +499,6 @@ k_build_lists(Grid g) {
 
 namespace pbfint {
 
-size_t sweep_tile_count(pbf_int3 dims) {
-    const size_t chunks = (static_cast<size_t>(dims.x) + kSweepTileW - 1) / kSweepTileW;
-    return chunks * static_cast<size_t>(dims.y) * static_cast<size_t>(dims.z);
-}
-
 pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t* nbr, int* cnt,
                              size_t stride, uint32_t* status, void* stream) {
     if (sg.n == 0) return PBF_OK;

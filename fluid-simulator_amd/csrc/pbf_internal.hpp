@@ -13,17 +13,13 @@ namespace pbfint {
 // Scenes of a batched step.  Scene s keeps its own grid (bounds, dims) and
 // container; its particles occupy the slots [first, end) in every order the
 // step produces (cell order sorts by global key = cellOff + the scene's own
-// key, so scenes stay in their slot ranges), and its sweep tiles are the
-// global tiles tileOff + (the scene's own tile index).
+// key, so scenes stay in their slot ranges).
 struct SceneGeom {
     pbf_float3 minBounds;
     pbf_int3 dims;
     uint32_t cellOff;
-    uint32_t tileOff;
-    uint32_t chunks;  // sweep tiles per grid row
     uint32_t first;
     uint32_t end;
-    uint32_t pad;
 };
 // Device tables of a batched step; all null for a single scene.
 struct SceneTable {
@@ -77,17 +73,12 @@ pbf_status launch_find_neighbors(const pbf_float4* p, const int* cs, const int* 
                                  uint32_t* nbr, int* cnt, int maxN, size_t stride,
                                  uint32_t* flag, uint32_t bit, const uint32_t* guard,
                                  void* stream);
-// Cells along x of one neighbour-sweep tile.
-constexpr uint32_t kSweepTileW = 32;
 // -1 into cellStart and, unless ce is null, cellEnd.
 pbf_status launch_fill_cells(int* cs, int* ce, size_t cells, void* stream);
-// Cell ranges of the sorted keys; with tileEnd not null, also the end of
-// every non-empty sweep tile (sweep_tile_count(dims) entries, only those of
-// non-empty tiles written).
-// With splitters not null, also the keys at multiples of the re-sort tile
-// (resort_splitter_count(n) entries) for the next resort_pairs.
-pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                              uint32_t* tileEnd, const SceneTable& st, void* stream,
+// Cell ranges of the sorted keys.  With splitters not null, also the keys
+// at multiples of the re-sort tile (resort_splitter_count(n) entries) for
+// the next resort_pairs.
+pbf_status launch_cell_ranges(const uint32_t* sk, int* cs, int* ce, size_t n, void* stream,
                               uint32_t* splitters = nullptr);
 // The working-set permutation of a grid build: slot i takes the records of
 // slot idx[i].  With x null only p* moves, and permOut[i] = permIn[idx[i]]
@@ -103,9 +94,8 @@ struct Permute {
     const uint32_t* permIn = nullptr;
     uint32_t* permOut = nullptr;
 };
-// launch_cell_ranges (tileEnd required) and the permutation in one pass.
-pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, pbf_int3 dims,
-                                uint32_t* tileEnd, const SceneTable& st, const Permute& pm,
+// launch_cell_ranges and the permutation in one pass.
+pbf_status launch_ranges_gather(const uint32_t* sk, int* cs, int* ce, size_t n, const Permute& pm,
                                 void* stream, uint32_t* splitters = nullptr);
 size_t radix_scratch_words(size_t n);
 // Scratch of a grid of capacity n: the radix sort's words, then the
@@ -252,14 +242,13 @@ pbf_status tape_records(const pbf_float4* a, pbf_float4* aOut, const pbf_float4*
                         pbf_float4* bOut, const pbf_float4* c, pbf_float4* cOut, size_t n,
                         void* stream);
 
-// Neighbour lists by LDS-tiled cell sweep (k_sweep.hip).  The grid must
+// Neighbour lists by LDS-staged cell sweep (k_sweep.hip).  The grid must
 // have cell size >= h (a 3x3x3 stencil) and the working set must be in the
-// order of sortedKeys; tileEnd comes from launch_cell_ranges.
+// order of sortedKeys.
 struct SweepGrid {
     const int* cellStart;
     const int* cellEnd;
     const uint32_t* sortedKeys;
-    const uint32_t* tileEnd;
     pbf_int3 dims;  // one scene's grid (unused when st is set)
     size_t n;
     float h;
@@ -273,8 +262,6 @@ struct SweepGrid {
     const pbf_container* box = nullptr;
     float radius = 0.0f;
 };
-// Entries of the tileEnd array for a grid of these dims.
-size_t sweep_tile_count(pbf_int3 dims);
 // The lists launch_find_neighbors builds (at most maxNeighbors stored,
 // counts whole), bit for bit, in grouped layout: entry q of particle i at
 // word 4 * ((q / 4) * stride + i) + q % 4, and the slots of the last group
