@@ -322,6 +322,11 @@ pbf_status pbf_solver_set_incremental_sort(pbf_solver* s, int enabled) {
     return guarded([&] { s->solver.setIncrementalSort(enabled != 0); });
 }
 
+pbf_status pbf_solver_set_fused_xsph(pbf_solver* s, int enabled) {
+    PBF_REQUIRE(s);
+    return guarded([&] { s->solver.setFusedXsph(enabled != 0); });
+}
+
 pbf_status pbf_solver_set_neighbor_skin(pbf_solver* s, float skin) {
     PBF_REQUIRE(s);
     return guarded([&] { s->solver.setNeighborSkin(skin); });
@@ -443,6 +448,11 @@ pbf_status pbf_batch_set_graphs(pbf_batch* b, int enabled) {
 pbf_status pbf_batch_set_incremental_sort(pbf_batch* b, int enabled) {
     PBF_REQUIRE(b);
     return guarded([&] { b->batch.setIncrementalSort(enabled != 0); });
+}
+
+pbf_status pbf_batch_set_fused_xsph(pbf_batch* b, int enabled) {
+    PBF_REQUIRE(b);
+    return guarded([&] { b->batch.setFusedXsph(enabled != 0); });
 }
 
 pbf_status pbf_batch_set_neighbor_skin(pbf_batch* b, float skin) {

@@ -324,6 +324,8 @@ struct PBFSolver::State {
     // Grid builds re-sort from the previous build when the particles are in
     // its order (the same result as a full sort, bit for bit).
     bool resortBuilds = true;
+    // XSPH runs inside the post-solve build (the same results bit for bit).
+    bool fusedXsph = true;
     // Nothing is known of the order of the particles (new particles, a
     // restored checkpoint): the next builds sort in full.
     void forget_order() {
@@ -569,6 +571,20 @@ struct PBFSolver::State {
         launch_check(pbfint::sweep_build_lists(sweep_grid(sk), p.data(), nbr.data(), cnt.data(),
                                                stride, status.data(), stream));
     }
+    // The post-solve build with XSPH: the lists, and the velocities after
+    // applyXsphViscosity into va (and the tape's copy when taping).
+    void build_lists_xsph(float visc, float invDt, float4* tapeOut) {
+        Scope sc(prof, "build_lists_xsph", stream);
+        pbfint::SweepGrid g = sweep_grid();
+        g.xs = x.data();
+        g.vold = v.data();
+        g.vout = va.data();
+        g.tapeOut = tapeOut;
+        g.visc = visc;
+        g.inv_dt = invDt;
+        launch_check(pbfint::sweep_build_lists(g, p.data(), nbr.data(), cnt.data(), stride,
+                                               status.data(), stream));
+    }
 
     // One frame: params.substeps substeps (pbf_solver.cu:186-325).
     void enqueue_frame() {
@@ -743,15 +759,23 @@ struct PBFSolver::State {
                 }
                 std::swap(p, pa);
             }
+            const bool fuseXsph = fusedXsph && P.xsphViscosity != 0.0f && post;
             if (post) {
                 build_and_reorder();
-                build_lists();
+                if (fuseXsph) {
+                    SubstepTape* tp = tape();
+                    build_lists_xsph(P.xsphViscosity, invDt, tp != nullptr ? tp->v3 : nullptr);
+                } else {
+                    build_lists();
+                }
                 tape_build(P.solverIterations);
                 if (SubstepTape* tp = tape())
                     launch_check(pbfint::tape_records(x.data(), tp->xs, p.data(), tp->pf,
                                                       vin.data(), tp->vin, n, stream));
             }
-            if (P.xsphViscosity != 0.0f) {
+            if (fuseXsph) {
+                std::swap(v, va);  // the build wrote the velocities
+            } else if (P.xsphViscosity != 0.0f) {
                 Scope sc(prof, "update_xsph", stream);
                 pbfint::ListPass xl = lpass();
                 if (SubstepTape* tp = tape()) xl.tapeOut = tp->v3;
@@ -1345,6 +1369,11 @@ void PBFSolver::setIncrementalSort(bool enabled) {
     state_->drop_graphs();
 }
 
+void PBFSolver::setFusedXsph(bool enabled) {
+    state_->fusedXsph = enabled;
+    state_->drop_graphs();
+}
+
 void PBFSolver::setProfiling(bool enabled) {
     state_->prof.enabled = enabled;
     state_->prof.reset();
@@ -1495,6 +1524,7 @@ std::size_t PBFBatchSolver::sceneFirst(std::size_t scene) const {
 
 void PBFBatchSolver::setGraphs(bool enabled) { solver_.setGraphs(enabled); }
 void PBFBatchSolver::setIncrementalSort(bool enabled) { solver_.setIncrementalSort(enabled); }
+void PBFBatchSolver::setFusedXsph(bool enabled) { solver_.setFusedXsph(enabled); }
 void PBFBatchSolver::setNeighborSkin(float skin) { solver_.setNeighborSkin(skin); }
 std::size_t PBFBatchSolver::staleFrames() const { return solver_.staleFrames(); }
 void PBFBatchSolver::setTuning(const PBFSolver::Tuning& tuning) { solver_.setTuning(tuning); }

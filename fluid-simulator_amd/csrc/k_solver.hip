@@ -637,14 +637,6 @@ __global__ void __launch_bounds__(kBlock)
     });
 }
 
-// (p* - x) / dt as the reference's velocity update stores it
-// (pbf_solver_kernels.cu:207): rounded, never fused into the XSPH sums that
-// read it back.
-__device__ __forceinline__ float stored_velocity(float pc, float xc, float inv_dt) {
-#pragma clang fp contract(off)
-    return (pc - xc) * inv_dt;
-}
-
 // updateVelocityAndPosition + applyXsphViscosity (pbf_solver.cu:284-298).
 // Velocities v = (p* - x) / dt are rebuilt on the fly for i and every
 // neighbour with the same FP32 expression the reference stores, so the
@@ -676,29 +668,21 @@ __global__ void __launch_bounds__(kBlock)
         return;
     }
     const Sph s = sph_consts(h);
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    const float3 pi3 = make_float3(pi.x, pi.y, pi.z);
+    const float3 vi3 = make_float3(vix, viy, viz);
+    float3 cs = make_float3(0.0f, 0.0f, 0.0f);
     int within = 0;
     for_each_neighbour<true>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &xj = r.b;
-            const float vjx = stored_velocity(pj.x, xj.x, inv_dt);
-            const float vjy = stored_velocity(pj.y, xj.y, inv_dt);
-            const float vjz = stored_velocity(pj.z, xj.z, inv_dt);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            float d2;
-            if (kSkin) {
-                d2 = dist2_exact(dx, dy, dz);
-                within += d2 <= s.h2 ? 1 : 0;
-            } else {
-                d2 = dx * dx + dy * dy + dz * dz;
-            }
-            const float w = poly6_d2(d2, s);
-            cx += (vjx - vix) * w;
-            cy += (vjy - viy) * w;
-            cz += (vjz - viz) * w;
+            const float3 vj = make_float3(stored_velocity(pj.x, xj.x, inv_dt),
+                                          stored_velocity(pj.y, xj.y, inv_dt),
+                                          stored_velocity(pj.z, xj.z, inv_dt));
+            if (kSkin) within += dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) <= s.h2;
+            xsph_term(pi3, vi3, make_float3(pj.x, pj.y, pj.z), vj, s, cs);
         });
-    const float4 vnew = make_float4(vix + visc * cx, viy + visc * cy, viz + visc * cz, vw);
+    const float4 vnew = make_float4(vix + visc * cs.x, viy + visc * cs.y, viz + visc * cs.z, vw);
     st4(vout, i, vnew);
     if (tapeOut != nullptr) st4(tapeOut, i, vnew);
     if (kSkin && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);

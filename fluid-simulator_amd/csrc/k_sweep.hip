@@ -82,6 +82,16 @@ struct Grid {
     pbf_container box;
     float radius;
     uint32_t fullBit;
+    // XSPH of the post-solve build (kXsph): start positions, velocities in
+    // (.w kept) and out, the tape's copy of the output (or null), the
+    // viscosity, 1 / dt and h.
+    const pbf_float4* xs;
+    const pbf_float4* vold;
+    pbf_float4* vout;
+    pbf_float4* tapeOut;
+    float visc;
+    float inv_dt;
+    float h;
 };
 
 // The grid a particle belongs to: the one grid of a single-scene step, or
@@ -113,11 +123,24 @@ __device__ __forceinline__ float3 test_point(const pbf_container& box, float rad
 // loads of a four-candidate step share one address register with immediate
 // offsets, and lanes ~8 records apart (neighbouring cells) land 24 banks
 // apart, where 16-byte records would share one bank group.
+// The XSPH build also stages each candidate's velocity (v = (p* - x) / dt)
+// beside its position.
+template <bool kXsph>
 struct WaveLds {
     float xyz[3 * (kStage + kPad)];
+    float vel[kXsph ? 3 * (kStage + kPad) : 1];
     uint16_t off[kPieces][kSegs][kSegCells + 1];  // exclusive prefix of per-cell counts
     int first[kPieces][kSegs];  // working-set index of the segment's start
     uint32_t buf[kBufRows][64];  // row r of lane l at buf[r][l]
+};
+
+// A neighbour's XSPH inputs: its p* and velocity.
+struct XRec {
+    float3 p, v;
+};
+// The lane's XSPH sum: its own p* and velocity, the sum so far.
+struct XsphSum {
+    float3 pi, vi, c;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
@@ -142,10 +165,15 @@ __device__ __forceinline__ void wave_sync() {
 // (padding in LDS, clamped in global memory).
 struct LdsRow {
     const float* xyz;
+    const float* vel;
     int base;
     __device__ float3 position(int q) const {
         const float* r = xyz + 3 * (base + q);
         return make_float3(r[0], r[1], r[2]);
+    }
+    __device__ XRec record(int q, uint32_t) const {
+        const float* v = vel + 3 * (base + q);
+        return XRec{position(q), make_float3(v[0], v[1], v[2])};
     }
 };
 
@@ -156,8 +184,17 @@ struct GlobalRow {
     int last;
     pbf_container box;
     float radius;
+    const pbf_float4* xs;  // XSPH: start positions
+    float inv_dt;
     __device__ float3 position(int q) const {
         return test_point<kSkin>(box, radius, ld4(p, first + static_cast<size_t>(min(q, last))));
+    }
+    __device__ XRec record(int, uint32_t j) const {
+        const float4 pj = ld4(p, j), xj = ld4(xs, j);
+        return XRec{make_float3(pj.x, pj.y, pj.z),
+                    make_float3(stored_velocity(pj.x, xj.x, inv_dt),
+                                stored_velocity(pj.y, xj.y, inv_dt),
+                                stored_velocity(pj.z, xj.z, inv_dt))};
     }
 };
 
@@ -201,8 +238,14 @@ struct ListState {
 // are garbage and land beyond its open group) and writes them to rows
 // open .. open + 3 of the lane's column; when the open group (rows 0 .. 3)
 // fills, it is stored and the entries past it move down four rows.
-__device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t base,
-                                       ListState& st, WaveLds& L, int lane) {
+//
+// The XSPH build (kXsph) adds each kept entry's XSPH term, in list order,
+// from the candidate's record (src.record(q), q = qbase + lz its index in
+// the segment).
+template <bool kXsph, class Src>
+__device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t base, int qbase,
+                                       ListState& st, WaveLds<kXsph>& L, int lane,
+                                       const Src& src, XsphSum& xs, const Sph& sph) {
     int hits = __builtin_popcount(mask);
     st.count += hits;
     const int room = g.maxNeighbors - st.stored;
@@ -214,11 +257,21 @@ __device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t ba
     }
     for (int rem = hits; rem > 0; rem -= 4) {
         uint32_t e[4];
+        int lzs[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int lz = __builtin_clzg(mask, -1);
+            lzs[u] = lz;
             e[u] = base + static_cast<uint32_t>(lz);
             mask &= 0x7fffffffu >> (lz & 31);
+        }
+        if (kXsph) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < rem) {
+                    const XRec r = src.record(qbase + lzs[u], e[u]);
+                    xsph_term(xs.pi, xs.vi, r.p, r.v, sph, xs.c);
+                }
         }
         uint32_t* col = &L.buf[st.open][lane];
 #pragma unroll
@@ -246,10 +299,11 @@ __device__ __forceinline__ void append(const Grid& g, uint32_t mask, uint32_t ba
 // Appends the neighbours of the particle at test point ip among candidates
 // [lo, hi) of one segment (first = the segment's first working-set index;
 // selfq = the particle's own index in the segment, or -1) to its list.
-template <class Src>
+template <bool kXsph, class Src>
 __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 ip, int lo, int hi,
-                                          int selfq, size_t first, ListState& st, WaveLds& L,
-                                          int lane) {
+                                          int selfq, size_t first, ListState& st,
+                                          WaveLds<kXsph>& L, int lane, XsphSum& xs,
+                                          const Sph& sph) {
     for (int c = lo; c < hi; c += 32) {
         const int w = min(32, hi - c);
         uint32_t mask = 0u;
@@ -270,15 +324,18 @@ __device__ __forceinline__ void sweep_row(const Grid& g, const Src& src, float3 
         const int sq = selfq - c;
         if (static_cast<unsigned>(sq) < static_cast<unsigned>(w)) mask &= ~(1u << (w4 - 1 - sq));
         if (mask == 0u) continue;
-        append(g, mask, static_cast<uint32_t>(first) + static_cast<uint32_t>(c + w4 - 32), st, L,
-               lane);
+        append<kXsph>(g, mask, static_cast<uint32_t>(first) + static_cast<uint32_t>(c + w4 - 32),
+                      c + w4 - 32, st, L, lane, src, xs, sph);
     }
 }
 
 // Stores the open group, its free slots repeating the last entry, and the
-// count.
+// count; the XSPH build also the velocity (NaN for an invalid count, as
+// applyXsphViscosity writes, pbf_solver_kernels.cu:225-233).
+template <bool kXsph>
 __device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const ListState& st,
-                                                const WaveLds& L, int lane) {
+                                                const WaveLds<kXsph>& L, int lane,
+                                                const XsphSum& xs, float vw) {
     if (st.open > 0) {
         const uint32_t a = L.buf[0][lane];
         const uint32_t b = st.open > 1 ? L.buf[1][lane] : a;
@@ -287,14 +344,24 @@ __device__ __forceinline__ void finish_particle(const Grid& g, size_t gi, const 
     }
     g.cnt[gi] = st.count;
     if (st.count > g.maxNeighbors) atomicOr(g.status, g.fullBit);
+    if (kXsph) {
+        const float4 out =
+            valid_count(st.count, g.maxNeighbors)
+                ? make_float4(xs.vi.x + g.visc * xs.c.x, xs.vi.y + g.visc * xs.c.y,
+                              xs.vi.z + g.visc * xs.c.z, vw)
+                : make_float4(qnan(), qnan(), qnan(), vw);
+        st4(g.vout, gi, out);
+        if (g.tapeOut != nullptr) st4(g.tapeOut, gi, out);
+    }
 }
 
 // One particle on one lane, candidates read from global memory: the row
 // range of every (dz, dy) is the run of particles from the first non-empty
 // cell of cx-1 .. cx+1 to the last.
-template <bool kSkin>
+template <bool kSkin, bool kXsph>
 __device__ __forceinline__ void sweep_lane(const Grid& g, const Geom& gm, size_t gi, uint32_t key,
-                                           ListState& st, WaveLds& L, int lane) {
+                                           ListState& st, WaveLds<kXsph>& L, int lane,
+                                           XsphSum& xs, const Sph& sph) {
     const uint32_t gx = static_cast<uint32_t>(gm.dims.x);
     const int cx = static_cast<int>(key % gx);
     const uint32_t rowId = key / gx;
@@ -318,8 +385,10 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, const Geom& gm, size_t
         if (rowStart < 0) continue;
         const int len = rowEnd - rowStart;
         const int selfq = s == 4 ? static_cast<int>(gi - static_cast<size_t>(rowStart)) : -1;
-        const GlobalRow<kSkin> src{g.p, static_cast<size_t>(rowStart), len - 1, gm.box, g.radius};
-        sweep_row(g, src, ip, 0, len, selfq, static_cast<size_t>(rowStart), st, L, lane);
+        const GlobalRow<kSkin> src{g.p, static_cast<size_t>(rowStart), len - 1, gm.box, g.radius,
+                                   g.xs, g.inv_dt};
+        sweep_row<kXsph>(g, src, ip, 0, len, selfq, static_cast<size_t>(rowStart), st, L, lane, xs,
+                         sph);
     }
 }
 
@@ -327,14 +396,14 @@ __device__ __forceinline__ void sweep_lane(const Grid& g, const Geom& gm, size_t
 // Eight workgroups fit a CU's LDS (4.6 KB per wave), eight waves per SIMD:
 // more waves hide more of the rows' staging loads (measured: 6, 7 and 8
 // waves per SIMD give 1047, 992 and 966 us per launch on the 8M double dam
-// break).
-template <bool kBatched, bool kSkin>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// break).  The XSPH build's staged velocities leave room for six.
+template <bool kBatched, bool kSkin, bool kXsph>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kXsph ? 6 : 8, 8)))
 k_build_lists(Grid g) {
-    __shared__ WaveLds ldsAll[kWaves];
+    __shared__ WaveLds<kXsph> ldsAll[kWaves];
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const int wv = static_cast<int>(threadIdx.x) >> 6;
-    WaveLds& L = ldsAll[wv];
+    WaveLds<kXsph>& L = ldsAll[wv];
     const size_t w0 = (static_cast<size_t>(xcd_task(static_cast<int>(blockIdx.x),
                                                     static_cast<int>(gridDim.x))) *
                            kWaves + static_cast<size_t>(wv)) * 64;
@@ -356,6 +425,18 @@ k_build_lists(Grid g) {
     const int cz = static_cast<int>(rowId / static_cast<uint32_t>(gm.dims.y));
     // The key of cell x = 0 of the particle's row names the row across scenes.
     const uint32_t rowKey = skey - static_cast<uint32_t>(cx);
+    // XSPH: the lane's own p* and velocity (k_update_xsph's inputs).
+    const Sph sph = sph_consts(g.h);
+    XsphSum xv{};
+    float vw = 0.0f;
+    if (kXsph) {
+        const float4 pp = ld4(g.p, gc), xx = ld4(g.xs, gc);
+        xv.pi = make_float3(pp.x, pp.y, pp.z);
+        xv.vi = make_float3(stored_velocity(pp.x, xx.x, g.inv_dt),
+                            stored_velocity(pp.y, xx.y, g.inv_dt),
+                            stored_velocity(pp.z, xx.z, g.inv_dt));
+        vw = g.vold[gc].w;
+    }
 
     // Pieces: runs of lanes in one grid row.
     const uint32_t prevRow = __shfl_up(rowKey, 1, 64);
@@ -370,8 +451,8 @@ k_build_lists(Grid g) {
     ListState st{g.nbr + gi, 0, 0, 0};
     if (pieces > kPieces || wide) {
         if (active) {
-            sweep_lane<kSkin>(g, gm, gi, key, st, L, lane);
-            finish_particle(g, gi, st, L, lane);
+            sweep_lane<kSkin, kXsph>(g, gm, gi, key, st, L, lane, xv, sph);
+            finish_particle<kXsph>(g, gi, st, L, lane, xv, vw);
         }
         return;
     }
@@ -425,8 +506,8 @@ k_build_lists(Grid g) {
     wave_sync();
     if (__ballot(deep) != 0ull) {
         if (active) {
-            sweep_lane<kSkin>(g, gm, gi, key, st, L, lane);
-            finish_particle(g, gi, st, L, lane);
+            sweep_lane<kSkin, kXsph>(g, gm, gi, key, st, L, lane, xv, sph);
+            finish_particle<kXsph>(g, gi, st, L, lane, xv, vw);
         }
         return;
     }
@@ -454,11 +535,15 @@ k_build_lists(Grid g) {
             const long long f0 = L.first[0][s];
             const long long f1 = pieces > 1 ? static_cast<long long>(L.first[1][s]) - len0 : 0;
             for (int q0 = lane; q0 < total; q0 += 64 * kStageUnroll) {
-                float4 v[kStageUnroll];
+                float4 v[kStageUnroll], x[kXsph ? kStageUnroll : 1];
 #pragma unroll
                 for (int u = 0; u < kStageUnroll; ++u) {
                     const int q = q0 + 64 * u;
-                    if (q < total) v[u] = ld4(g.p, static_cast<size_t>((q < len0 ? f0 : f1) + q));
+                    if (q < total) {
+                        const size_t at = static_cast<size_t>((q < len0 ? f0 : f1) + q);
+                        v[u] = ld4(g.p, at);
+                        if (kXsph) x[kXsph ? u : 0] = ld4(g.xs, at);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kStageUnroll; ++u) {
@@ -469,6 +554,13 @@ k_build_lists(Grid g) {
                         o[0] = r.x;
                         o[1] = r.y;
                         o[2] = r.z;
+                        if (kXsph) {
+                            const float4 xq = x[kXsph ? u : 0];
+                            float* w = L.vel + 3 * q;
+                            w[0] = stored_velocity(v[u].x, xq.x, g.inv_dt);
+                            w[1] = stored_velocity(v[u].y, xq.y, g.inv_dt);
+                            w[2] = stored_velocity(v[u].z, xq.z, g.inv_dt);
+                        }
                     }
                 }
             }
@@ -481,18 +573,18 @@ k_build_lists(Grid g) {
                 const size_t first = static_cast<size_t>(L.first[piece][s]);
                 const int selfq = s == 4 ? static_cast<int>(gi - first) : -1;
                 if (staged) {
-                    const LdsRow src{L.xyz, piece != 0 ? len0 : 0};
-                    sweep_row(g, src, ip, lo, hi, selfq, first, st, L, lane);
+                    const LdsRow src{L.xyz, L.vel, piece != 0 ? len0 : 0};
+                    sweep_row<kXsph>(g, src, ip, lo, hi, selfq, first, st, L, lane, xv, sph);
                 } else {
-                    const GlobalRow<kSkin> src{g.p, first, (piece != 0 ? len1 : len0) - 1, gm.box,
-                                               g.radius};
-                    sweep_row(g, src, ip, lo, hi, selfq, first, st, L, lane);
+                    const GlobalRow<kSkin> src{g.p,      first,  (piece != 0 ? len1 : len0) - 1,
+                                               gm.box,   g.radius, g.xs, g.inv_dt};
+                    sweep_row<kXsph>(g, src, ip, lo, hi, selfq, first, st, L, lane, xv, sph);
                 }
             }
         }
         wave_sync();  // the row's readers are done before the next row is staged
     }
-    if (active) finish_particle(g, gi, st, L, lane);
+    if (active) finish_particle<kXsph>(g, gi, st, L, lane, xv, vw);
 }
 
 }  // namespace
@@ -528,17 +620,31 @@ pbf_status sweep_build_lists(const SweepGrid& sg, const pbf_float4* p, uint32_t*
     g.box = sg.box != nullptr ? *sg.box : pbf_container{};
     g.radius = sg.radius;
     g.fullBit = sg.skin ? PBF_STATUS_LIST_STALE : PBF_STATUS_NEIGHBOR_OVERFLOW;
+    g.xs = sg.xs;
+    g.vold = sg.vold;
+    g.vout = sg.vout;
+    g.tapeOut = sg.tapeOut;
+    g.visc = sg.visc;
+    g.inv_dt = sg.inv_dt;
+    g.h = sg.h;
+    if (sg.vout != nullptr && (sg.xs == nullptr || sg.vold == nullptr))
+        return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "the XSPH build needs x and the velocities");
     if (sg.skin && sg.st.geom == nullptr && sg.box == nullptr)
         return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "skin lists need the container");
     const size_t tasks = (sg.n + kThreads - 1) / kThreads;
     if (tasks > 0x7fffffffull)
         return pbf_fail(PBF_ERR_LENGTH, "too many particles for the neighbour sweep");
     const unsigned nb = static_cast<unsigned>(tasks);
-    if (sg.st.geom != nullptr && sg.skin)
-        PBF_LAUNCH((k_build_lists<true, true>), nb, kThreads, stream, g);
-    if (sg.st.geom != nullptr) PBF_LAUNCH((k_build_lists<true, false>), nb, kThreads, stream, g);
-    if (sg.skin) PBF_LAUNCH((k_build_lists<false, true>), nb, kThreads, stream, g);
-    PBF_LAUNCH((k_build_lists<false, false>), nb, kThreads, stream, g);
+    const bool batched = sg.st.geom != nullptr;
+    if (sg.vout != nullptr) {
+        if (sg.skin) return pbf_fail(PBF_ERR_INVALID_ARGUMENT, "XSPH is fused into exact builds only");
+        if (batched) PBF_LAUNCH((k_build_lists<true, false, true>), nb, kThreads, stream, g);
+        PBF_LAUNCH((k_build_lists<false, false, true>), nb, kThreads, stream, g);
+    }
+    if (batched && sg.skin) PBF_LAUNCH((k_build_lists<true, true, false>), nb, kThreads, stream, g);
+    if (batched) PBF_LAUNCH((k_build_lists<true, false, false>), nb, kThreads, stream, g);
+    if (sg.skin) PBF_LAUNCH((k_build_lists<false, true, false>), nb, kThreads, stream, g);
+    PBF_LAUNCH((k_build_lists<false, false, false>), nb, kThreads, stream, g);
 }
 
 }  // namespace pbfint

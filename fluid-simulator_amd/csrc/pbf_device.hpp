@@ -90,6 +90,14 @@ __device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// (p* - x) / dt as the reference's velocity update stores it
+// (pbf_solver_kernels.cu:207): rounded, never fused into the XSPH sums that
+// read it back.
+__device__ __forceinline__ float stored_velocity(float pc, float xc, float inv_dt) {
+#pragma clang fp contract(off)
+    return (pc - xc) * inv_dt;
+}
+
 // ---------------------------------------------------------------------------
 // SPH kernels (include/pbf/sph_helpers.cuh:13-95)
 // ---------------------------------------------------------------------------
@@ -122,6 +130,22 @@ __device__ __forceinline__ float poly6_d2(float d2, const Sph& s) {
     if (d2 >= s.h2) return 0.0f;
     const float rd = (s.h2 - d2) * s.inv_h2;
     return s.poly_norm * (rd * rd * rd);
+}
+
+// One neighbour's XSPH term (applyXsphViscosity, pbf_solver_kernels.cu:
+// 242-257): c += (v_j - v_i) W(p*_i - p*_j).
+// The squared distance is spelled out as fma(dz, dz, fma(dx, dx, dy * dy)):
+// left to the compiler, dx * dx + dy * dy may be contracted around either
+// product, and it picks differently in different inlined copies, while the
+// list build's XSPH and the separate pass must agree bit for bit.
+__device__ __forceinline__ void xsph_term(float3 pi, float3 vi, float3 pj, float3 vj, const Sph& s,
+                                          float3& c) {
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+    const float w = poly6_d2(d2, s);
+    c.x += (vj.x - vi.x) * w;
+    c.y += (vj.y - vi.y) * w;
+    c.z += (vj.z - vi.z) * w;
 }
 
 // spikyGradientFromDistanceSquared: zero for r <= 0 or r >= h with

@@ -261,6 +261,17 @@ struct SweepGrid {
     bool skin = false;
     const pbf_container* box = nullptr;
     float radius = 0.0f;
+    // XSPH fused into the build (exact lists only): with vout set, every
+    // particle's velocity after applyXsphViscosity (the arithmetic of
+    // launch_update_xsph, its terms added in list order as the list is
+    // built) into vout and, unless null, tapeOut; xs holds the start
+    // positions (v = (p - xs) / dt), vold the velocities (.w kept).
+    const pbf_float4* xs = nullptr;
+    const pbf_float4* vold = nullptr;
+    pbf_float4* vout = nullptr;
+    pbf_float4* tapeOut = nullptr;
+    float visc = 0.0f;
+    float inv_dt = 0.0f;
 };
 // The lists launch_find_neighbors builds (at most maxNeighbors stored,
 // counts whole), bit for bit, in grouped layout: entry q of particle i at

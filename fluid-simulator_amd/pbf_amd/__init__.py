@@ -250,6 +250,7 @@ _SIGNATURES = {
     "pbf_solver_set_profiling": (_st, [_vp, _i]),
     "pbf_solver_set_graphs": (_st, [_vp, _i]),
     "pbf_solver_set_incremental_sort": (_st, [_vp, _i]),
+    "pbf_solver_set_fused_xsph": (_st, [_vp, _i]),
     "pbf_solver_set_neighbor_skin": (_st, [_vp, C.c_float]),
     "pbf_solver_neighbor_skin": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(_sz)]),
     "pbf_solver_set_tuning": (_st, [_vp, _vp]),
@@ -274,6 +275,7 @@ _SIGNATURES = {
     "pbf_batch_counts": (_st, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
     "pbf_batch_set_graphs": (_st, [_vp, _i]),
     "pbf_batch_set_incremental_sort": (_st, [_vp, _i]),
+    "pbf_batch_set_fused_xsph": (_st, [_vp, _i]),
     "pbf_batch_set_neighbor_skin": (_st, [_vp, C.c_float]),
     "pbf_batch_stale_frames": (_st, [_vp, C.POINTER(_sz)]),
     "pbf_batch_set_tuning": (_st, [_vp, _vp]),
@@ -696,6 +698,11 @@ class PBFSolver:
         full radix sort; the results are bitwise the same."""
         _check(lib().pbf_solver_set_incremental_sort(self._h, 1 if enabled else 0))
 
+    def setFusedXsph(self, enabled: bool):
+        """XSPH inside the post-solve grid build (default) or as a pass of its
+        own; the results are bitwise the same."""
+        _check(lib().pbf_solver_set_fused_xsph(self._h, 1 if enabled else 0))
+
     def setNeighborSkin(self, skin: float):
         """Skin lists: one grid build per substep, radius h * (1 + skin); 0 runs
         the reference's build before every iteration and after the solve."""
@@ -819,6 +826,9 @@ class PBFBatchSolver:
 
     def setIncrementalSort(self, enabled: bool):
         _check(lib().pbf_batch_set_incremental_sort(self._h, 1 if enabled else 0))
+
+    def setFusedXsph(self, enabled: bool):
+        _check(lib().pbf_batch_set_fused_xsph(self._h, 1 if enabled else 0))
 
     def setNeighborSkin(self, skin: float):
         _check(lib().pbf_batch_set_neighbor_skin(self._h, float(skin)))

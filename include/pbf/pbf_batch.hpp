@@ -54,6 +54,7 @@ public:
 
     void setGraphs(bool enabled);
     void setIncrementalSort(bool enabled);  // PBFSolver::setIncrementalSort
+    void setFusedXsph(bool enabled);        // PBFSolver::setFusedXsph
     // Skin lists for every scene (PBFSolver::setNeighborSkin).  A frame in
     // which any scene's lists go stale runs again for every scene with the
     // reference's builds.

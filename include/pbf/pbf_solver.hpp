@@ -62,6 +62,10 @@ public:
     // its order (default on; see pbf_solver_set_incremental_sort).  Results
     // are bitwise the same either way.  No counterpart in the reference.
     void setIncrementalSort(bool enabled);
+    // XSPH fused into the post-solve grid build (default on; see
+    // pbf_solver_set_fused_xsph).  Results are bitwise the same either way.
+    // No counterpart in the reference.
+    void setFusedXsph(bool enabled);
     // Skin lists: one grid build per substep instead of one per constraint
     // iteration plus the post-solve rebuild (see
     // pbf_solver_set_neighbor_skin).  skin is a fraction of the smoothing

@@ -411,6 +411,15 @@ pbf_status pbf_solver_set_graphs(pbf_solver* solver, int enabled);
  * bitwise the same either way; the option exists to compare the two. */
 pbf_status pbf_solver_set_incremental_sort(pbf_solver* solver, int enabled);
 
+/* XSPH fused into the post-solve grid build (enabled = 1, the default) or
+ * run as a pass of its own after it (0).  No counterpart in the reference,
+ * which runs applyXsphViscosity over the rebuilt lists
+ * (pbf_solver.cu:284-298).  The fused build adds each neighbour's XSPH term
+ * as it appends the neighbour to the list, with the arithmetic of the
+ * separate pass and in list order, so results are bitwise the same either
+ * way; the option exists to compare the two. */
+pbf_status pbf_solver_set_fused_xsph(pbf_solver* solver, int enabled);
+
 /* Skin lists (Verlet lists), no counterpart in the reference.  With skin > 0
  * a substep builds the grid and the neighbour lists once, instead of before
  * every constraint iteration and after the solve (pbf_solver.cu:210-282):
@@ -508,6 +517,7 @@ pbf_status pbf_batch_copy_velocities_to_host(pbf_batch* batch, pbf_float4* h_out
 pbf_status pbf_batch_counts(const pbf_batch* batch, size_t* sceneCount, size_t* particleCount);
 pbf_status pbf_batch_set_graphs(pbf_batch* batch, int enabled);
 pbf_status pbf_batch_set_incremental_sort(pbf_batch* batch, int enabled);
+pbf_status pbf_batch_set_fused_xsph(pbf_batch* batch, int enabled);
 /* Skin lists for every scene of the batch (pbf_solver_set_neighbor_skin).
  * A frame in which any scene's lists go stale runs again for all scenes
  * with the reference's builds. */
