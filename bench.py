@@ -103,6 +103,9 @@ PHASE_BYTES = {
     # builds at 4 iterations: (4 * 48 + 148) / 5 = 68.
     "ranges_gather": 68,
     "build_lists": 20 + LIST,          # read p*, key; write the list
+    # The post-solve build with XSPH summed as the list is written: also
+    # read x, write v (the list is not read back).
+    "build_lists_xsph": 52 + LIST,
     "lambda": 20 + LIST,               # read p*, the list; write lambda
     "delta_apply_collide": 40 + LIST,  # read p*, lambda, the list; write p' and its key
     "update_xsph": 48 + LIST,          # read x, p*, the list; write v
@@ -126,8 +129,9 @@ RESORT_PER_PARTICLE = RESORT_BYTES + RESORT_CHANGED_BYTES * RESORT_CHANGED
 # Flop model: per neighbour of each pass (the FP32 operations of the
 # reference's expression trees) and 8 per candidate radius test.
 FLOPS_PER_NEIGHBOUR = {"lambda": 38, "delta_apply_collide": 37, "update_xsph": 22,
-                       "vorticity": 34, "confine_resolve": 32, "build_lists": 0}
-TESTING = {"build_lists"}  # passes that run the radius test
+                       "vorticity": 34, "confine_resolve": 32, "build_lists": 0,
+                       "build_lists_xsph": 22}
+TESTING = {"build_lists", "build_lists_xsph"}  # passes that run the radius test
 TRAFFIC_SUMMARY = REPO / "profiles" / "build_lists_dam_break_1M_traffic.md"
 TRAFFIC_KERNEL = "build_lists"
 COUNTER_SUMMARY = REPO / "profiles" / "list_passes_dam_break_1M_counters.md"
@@ -156,20 +160,22 @@ def alg_bytes(kernel: str, n: int, key_bits: int, cells: int,
 
 
 def step_bytes_per_particle(iterations: int, key_bits: int, cells: int, n: int,
-                            resort: bool = True) -> float:
+                            resort: bool = True, fused_xsph: bool = True) -> float:
     """Algorithmic bytes of one substep of this dataflow per particle: the
     reference's five grid builds (one per constraint iteration plus the
     post-solve rebuild), each a sort (a re-sort from the previous build, or
     with resort=False the radix sort), the ranges, the permute and the
     neighbour lists; the keys come from the prediction and the Delta-p
-    passes."""
+    passes.  With fused_xsph (the solver's default) the post-solve build
+    sums XSPH as it writes the list: x read and v written (32) in place of
+    the separate pass."""
     builds = iterations + 1
     sort = RESORT_PER_PARTICLE if resort else 16 * ((key_bits + 7) // 8)
     build = sort + 4 * cells / n + PHASE_BYTES["build_lists"]
     permute = RANGES_GATHER_DEFERRED * iterations + RANGES_GATHER_FULL
     return (PHASE_BYTES["predict_key"] + builds * build + permute
             + iterations * (PHASE_BYTES["lambda"] + PHASE_BYTES["delta_apply_collide"])
-            + PHASE_BYTES["update_xsph"] + PHASE_BYTES["vorticity"]
+            + (32 if fused_xsph else PHASE_BYTES["update_xsph"]) + PHASE_BYTES["vorticity"]
             + PHASE_BYTES["confine_resolve"])
 
 
