@@ -34,7 +34,14 @@
 //     nbr[k * stride + i] (16 bytes), so one store carries four entries and
 //     the list kernels read four entries with one load.  Slots of the last
 //     group past the count repeat its last entry, so every slot of a
-//     written group is a valid index.
+//     written group is a valid index;
+//   * the post-solve build of a step with XSPH (kXsph) also stages each
+//     candidate's velocity (p* - x) / dt beside its position, adds each
+//     appended neighbour's applyXsphViscosity term
+//     (src/pbf/pbf_solver_kernels.cu:213-264) to the lane's sum in list
+//     order, and stores the resulting velocity: the step then needs no
+//     separate XSPH pass, and the velocities are bitwise the separate
+//     pass's.
 //
 // Requires a 3x3x3 stencil (cell size >= the list radius), which the
 // solver's grids guarantee (src/pbf/pbf_solver.cu:81-82 for the reference's
