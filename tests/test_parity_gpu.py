@@ -147,9 +147,10 @@ def test_one_step_parity_dam_break_config1(gpu):
 def block_with_spray(clusters=600, seed=11):
     """A 16^3 dam-break block plus a spray of 3-particle clusters above it.
 
-    In sorted order the spray puts dozens of sparse cell tiles into one
-    256-particle range, which the solver sweeps with one lane per particle;
-    the block's tiles are swept in LDS.  Both forms run in every pass."""
+    In sorted order the spray's waves of 64 particles span many sparse grid
+    rows, which the list build sweeps with one lane per particle through
+    global memory; the block's waves stage their rows in LDS.  Both forms
+    run in every build."""
     sc = scenes.small_block(16, 16, 16)
     rng = np.random.default_rng(seed)
     centers = rng.uniform([1.0, 0.9, 0.2], [3.0, 1.4, 0.6], (clusters, 3))
@@ -179,7 +180,7 @@ def test_one_step_parity_block_with_spray(gpu):
 
 @pytest.mark.parametrize("scene_name", ["dam_break_1", "block_with_spray"])
 def test_step_neighbor_lists_equal_find_neighbors(gpu, oracle64, scene_name):
-    """The step builds its lists by LDS cell sweep (tile form and, for the
+    """The step builds its lists by LDS cell sweep (staged waves and, for the
     spray, the per-lane form); after two frames the last build's lists equal
     findNeighbors' on the same positions, entry by entry."""
     sc = scenes.dam_break(1) if scene_name == "dam_break_1" else block_with_spray()
