@@ -329,6 +329,42 @@ def test_literal_headline_strengths_overflow_like_the_oracle(gpu):
         g.step()
 
 
+def test_dense_cell_lists_past_16_bit_segment_offsets(gpu):
+    """66,000 particles in one grid cell: the list build's segment offsets
+    (16-bit) cannot hold the cell, so its waves take the per-lane sweep.
+    Every particle has thousands of neighbours, so the step raises the
+    reference's overflow; the lists of that build still hold each
+    particle's whole count and its first 256 neighbours in sorted order,
+    checked against the inclusive FP32 test (no FMA) on a sample.  Gravity
+    and velocities are zero, so p* is the input and one cell keeps the
+    input order."""
+    import pbf_amd
+
+    h = scenes.H
+    n = 66_000
+    rng = np.random.default_rng(5)
+    x = rng.uniform(2.05 * h, 2.95 * h, (n, 3)).astype(np.float32)
+    pos = np.c_[x, np.ones(n, np.float32)].astype(np.float32)
+    params = scenes.dam_break_params()
+    params.gravity = pbf_amd.float3(0.0, 0.0, 0.0)
+    s = pbf_amd.PBFSolver()
+    s.initialize(n, (0.0, 0.0, 0.0), (6 * h, 6 * h, 6 * h), params)
+    s.setParticles(pos, np.zeros_like(pos))
+    with pytest.raises(pbf_amd.NeighborOverflow):
+        s.step()
+    wpos, cnt, nbr = s.neighborLists(256)
+    assert np.array_equal(wpos[:, :3], x)
+    h2 = np.float32(h) * np.float32(h)
+    for i in np.r_[0, n - 1, rng.choice(n, 46, replace=False)]:
+        d = x - x[i]
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        within = d2 <= h2
+        within[i] = False
+        js = np.nonzero(within)[0].astype(np.uint32)
+        assert cnt[i] == len(js) > 256, i
+        assert np.array_equal(nbr[:, i], js[:256]), i
+
+
 def test_full_size_step_properties_config2(gpu, oracle64):
     sc = scenes.dam_break(2)
     s = _solver_for(sc, gpu.Solver)
