@@ -741,11 +741,15 @@ __global__ void __launch_bounds__(kBlock) k_cell_ranges(const uint32_t* __restri
 // The cell ranges of the sorted keys (k_cell_ranges) and, in the same
 // pass, the permutation of the working set into that order
 // (gatherSolverState, src/pbf/pbf_solver_kernels.cu:387-407): slot i of the
-// outputs takes the records of slot idx[i].
+// outputs takes the records of slot idx[i].  Workgroups take their slots
+// contiguously per XCD, as the passes around it do: a slot's record mostly
+// comes from a nearby slot that the same XCD wrote or reads next (1M dam
+// break: 15.7 against 17.5 us per launch with the plain order).
 __global__ void __launch_bounds__(kBlock)
     k_ranges_gather(const uint32_t* __restrict__ sk, int* __restrict__ cs, int* __restrict__ ce,
                     size_t n, pbfint::Permute pm, uint32_t* __restrict__ splitters) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i =
+        static_cast<size_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = pm.idx[i];
     st4(pm.po, i, ld4(pm.p, s));
