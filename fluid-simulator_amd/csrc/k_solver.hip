@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kBlock)
     double rho = static_cast<double>(mass * s.poly_norm);
     double gix = 0.0, giy = 0.0, giz = 0.0, sg = 0.0;
     int within = 0;
-    for_each_neighbour<kFast>(
+    for_each_neighbour<kFast, kFast ? 2 : 1>(
         nbr, stride, i, c, [&](uint32_t j) { return ld4(p, j); },
         [&](const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kBlock)
     const float4 vi = ld4(v, i);
     float ox = 0.0f, oy = 0.0f, oz = 0.0f;
     int within = 0;
-    for_each_neighbour<kFast>(
+    for_each_neighbour<kFast, kFast ? 2 : 1>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(v, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &vj = r.b;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void confine(size_t i, const pbf_float4* __restrict__
     const float4 wi = ld4(omega, i);
     const float wil = kFast ? pi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    for_each_neighbour<kFast>(
+    for_each_neighbour<kFast, kFast ? 2 : 1>(
         nbr, stride, i, c,
         [&](uint32_t j) {
             const float4 pj = ld4(p, j);
@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(kBlock)
         const Scorr sc = scorr_consts(h, sk, sn, sdq);
         const float li = pi.w;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_each_neighbour<true>(
+        for_each_neighbour<true, 2>(
             nbr, stride, i, c, [&](uint32_t j) { return ld4(pin, j); },
             [&](const float4& pj) {
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -672,7 +672,7 @@ __global__ void __launch_bounds__(kBlock)
     const float3 vi3 = make_float3(vix, viy, viz);
     float3 cs = make_float3(0.0f, 0.0f, 0.0f);
     int within = 0;
-    for_each_neighbour<true>(
+    for_each_neighbour<true, 2>(
         nbr, stride, i, c, [&](uint32_t j) { return PosPair{ld4(p, j), ld4(x, j)}; },
         [&](const PosPair& r) {
             const float4 &pj = r.a, &xj = r.b;

@@ -34,7 +34,7 @@ def _in_container(p, sc, tol=4e-6):
 
 def test_config3_one_substep_parity_with_fp64_oracle(gpu):
     from oracle import OracleSolver
-    from test_parity_gpu import _compare_with_contacts, _contact_mask, _solver_for
+    from test_parity_gpu import _compare_with_contacts, _contact_mask, _loose_misses, _solver_for
 
     sc = scenes.dam_break(3, substeps=1)
     t = time.perf_counter()
@@ -70,10 +70,15 @@ def test_config3_one_substep_parity_with_fp64_oracle(gpu):
     # The per-particle bars as in the one-step parity tests; the norm bar is
     # 1e-5 or, where an FP32 evaluation of the reference's own arithmetic
     # (the FP32 oracle, same input) is farther than that from FP64, that
-    # distance: measured 2.09e-5 for the FP32 oracle and 1.49e-5 for the GPU
-    # on this state.
+    # distance: measured 4.50e-5 for the FP32 oracle and 4.26e-5 for the GPU
+    # (the state is the GPU's own frame 5, so it moves with the GPU's
+    # rounding).
+    # Contact branches: the FP32 oracle takes the other branch than FP64 at
+    # a few contact particles of this state (measured 2; the GPU 1, one of
+    # the two), each then off by the whole branch; the GPU may flip as many.
+    f32_miss, _ = _loose_misses(*out["f32"], po, vo, sc)
     _compare_with_contacts(pg, vg, po, vo, sc, "double dam break 8M, frame 5",
-                           norm_bar=max(1e-5, f32_err))
+                           norm_bar=max(1e-5, f32_err), flips_allowed=int(f32_miss.sum()))
 
 
 def test_config3_frames_stay_inside_and_grid_is_bitexact(gpu, oracle64):

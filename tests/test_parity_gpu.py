@@ -223,12 +223,30 @@ def _contact_mask(p, sc, band=1e-3):
     return m
 
 
-def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True, norm_bar=1e-5):
+def _loose_misses(pg, vg, po, vo, sc):
+    """Particles beyond the 1e-3 bar (relative; the velocity floor of a few
+    position ulps per dt below that), and the contact mask."""
+    dt = sc.params.dt
+    dp = np.linalg.norm(pg[:, :3].astype(np.float64) - po[:, :3], axis=1)
+    pn = np.linalg.norm(po[:, :3].astype(np.float64), axis=1)
+    dv = np.linalg.norm(vg[:, :3].astype(np.float64) - vo[:, :3], axis=1)
+    vn = np.linalg.norm(vo[:, :3].astype(np.float64), axis=1)
+    ulp_x = np.spacing(np.abs(po[:, :3]).max(axis=1).astype(np.float32)).astype(np.float64)
+    loose = (dp <= 1e-3 * pn) & (dv <= np.maximum(1e-3 * vn, ULP_FLOOR * ulp_x / dt))
+    return ~loose, _contact_mask(po, sc) | _contact_mask(pg, sc)
+
+
+def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True, norm_bar=1e-5,
+                           flips_allowed=0):
     """Strict bar away from contacts; at contacts 1e-3 relative, at most 1%
     of the particles outside the strict bar; the norm bar over all.  With
     contacts_only=False a branch flip may also reach the flipped particle's
     neighbours through the later constraint iterations, so only the 1%
-    share, the 1e-3 bound and the norm bar apply."""
+    share, the 1e-3 bound and the norm bar apply.  flips_allowed contact
+    particles may pass the 1e-3 bound: a contact branch taken the other way
+    moves a particle by the whole branch, and the caller allows as many as
+    an FP32 evaluation of the reference's own arithmetic shows on the same
+    input."""
     dt = sc.params.dt
     dp = np.linalg.norm(pg[:, :3].astype(np.float64) - po[:, :3], axis=1)
     pn = np.linalg.norm(po[:, :3].astype(np.float64), axis=1)
@@ -236,16 +254,17 @@ def _compare_with_contacts(pg, vg, po, vo, sc, label, contacts_only=True, norm_b
     vn = np.linalg.norm(vo[:, :3].astype(np.float64), axis=1)
     ulp_x = np.spacing(np.abs(po[:, :3]).max(axis=1).astype(np.float32)).astype(np.float64)
     strict = (dp <= 1e-5 * pn) & (dv <= np.maximum(1e-5 * vn, ULP_FLOOR * ulp_x / dt))
-    contact = _contact_mask(po, sc) | _contact_mask(pg, sc)
-    loose = (dp <= 1e-3 * pn) & (dv <= np.maximum(1e-3 * vn, ULP_FLOOR * ulp_x / dt))
+    miss, contact = _loose_misses(pg, vg, po, vo, sc)
     norm_rel = np.linalg.norm(vg[:, :3] - vo[:, :3]) / max(np.linalg.norm(vo[:, :3]), 1e-30)
     outside = ~strict
     print(f"[{label}] contacts {contact.sum()}  outside strict bar {outside.sum()} "
-          f"(at contacts {np.sum(outside & contact)})  norm rel dv {norm_rel:.3e}", flush=True)
+          f"(at contacts {np.sum(outside & contact)})  beyond 1e-3 {miss.sum()} "
+          f"(allowed {flips_allowed} at contacts)  norm rel dv {norm_rel:.3e}", flush=True)
     assert np.all(np.isfinite(pg)) and np.all(np.isfinite(vg))
     if contacts_only:
         assert np.all(strict | contact), "a particle away from every contact misses the bar"
-    assert np.all(loose), "a particle differs by more than 1e-3"
+    assert not np.any(miss & ~contact), "a particle away from contacts differs by more than 1e-3"
+    assert miss.sum() <= flips_allowed, "more contact branches flipped than in FP32"
     assert outside.sum() <= max(1, len(pg) // 100)
     assert norm_rel <= norm_bar
     assert np.array_equal(pg[:, 3], po[:, 3]) and np.array_equal(vg[:, 3], vo[:, 3])
