@@ -146,17 +146,17 @@ __global__ void __launch_bounds__(kBlock)
                 d2 = dist2_exact(dx, dy, dz);
                 within += d2 <= s.h2 ? 1 : 0;
             } else {
-                d2 = dx * dx + dy * dy + dz * dz;
+                d2 = dist2_fma(dx, dy, dz);
             }
             rho += static_cast<double>(mass * poly6_d2(d2, s));
             const float3 gw = grad<kFast>(dx, dy, dz, d2, s);
             const float gjx = -gs * gw.x, gjy = -gs * gw.y, gjz = -gs * gw.z;
-            sg += static_cast<double>(gjx * gjx + gjy * gjy + gjz * gjz);
+            sg += static_cast<double>(dist2_fma(gjx, gjy, gjz));
             gix -= static_cast<double>(gjx);
             giy -= static_cast<double>(gjy);
             giz -= static_cast<double>(gjz);
         });
-    sg += gix * gix + giy * giy + giz * giz;
+    sg += dist2_fma(gix, giy, giz);
     const double C = rho / static_cast<double>(rho0) - 1.0;
     const float lam = static_cast<float>(-C / (sg + static_cast<double>(eps)));
     lambda[i * lstride] = lam;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t j = nbr[static_cast<size_t>(q) * stride + i];
         const float4 pj = ld4(p, j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float d2 = dist2_fma(dx, dy, dz);
         const float3 gw = spiky_grad(dx, dy, dz, d2, s);
         const float ls = li + lambda[j] + scorr_value(d2, s.h2, sc);
         ax += ls * gw.x;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kBlock)
         const float4 pj = ld4(p, j);
         const float4 vj = ld4(vin, j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float w = poly6_d2(dx * dx + dy * dy + dz * dz, s);
+        const float w = poly6_d2(dist2_fma(dx, dy, dz), s);
         cx += (vj.x - vi.x) * w;
         cy += (vj.y - vi.y) * w;
         cz += (vj.z - vi.z) * w;
@@ -293,14 +293,14 @@ __global__ void __launch_bounds__(kBlock)
             const float4 &pj = r.a, &vj = r.b;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             if (exactMax > 0) within += dist2_exact(dx, dy, dz) <= s.h2 ? 1 : 0;
-            const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+            const float3 g = grad<kFast>(dx, dy, dz, dist2_fma(dx, dy, dz), s);
             const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            ox += g.y * uz - g.z * uy;
-            oy += g.z * ux - g.x * uz;
-            oz += g.x * uy - g.y * ux;
+            ox += cross_fma(g.y, uz, g.z, uy);
+            oy += cross_fma(g.z, ux, g.x, uz);
+            oz += cross_fma(g.x, uy, g.y, ux);
         });
     if (exactMax > 0 && within > exactMax) atomicOr(status, PBF_STATUS_NEIGHBOR_OVERFLOW);
-    const float wl = kFast ? __builtin_sqrtf(ox * ox + oy * oy + oz * oz) : 0.0f;
+    const float wl = kFast ? __builtin_sqrtf(dist2_fma(ox, oy, oz)) : 0.0f;
     st4(omega, i, make_float4(ox, oy, oz, wl));
     if (tapeOut != nullptr) st4(tapeOut, i, make_float4(ox, oy, oz, wl));
     if (kFast) st4(pw, i, make_float4(pi.x, pi.y, pi.z, wl));
@@ -319,7 +319,7 @@ __device__ __forceinline__ void confine(size_t i, const pbf_float4* __restrict__
                                         float4& out) {
     const float4 pi = ld4(p, i);
     const float4 wi = ld4(omega, i);
-    const float wil = kFast ? pi.w : __builtin_sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z);
+    const float wil = kFast ? pi.w : __builtin_sqrtf(dist2_fma(wi.x, wi.y, wi.z));
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     for_each_neighbour<kFast, kFast ? 2 : 1>(
         nbr, stride, i, c,
@@ -330,29 +330,29 @@ __device__ __forceinline__ void confine(size_t i, const pbf_float4* __restrict__
                 wjl = pj.w;
             } else {
                 const float4 wj = ld4(omega, j);
-                wjl = __builtin_sqrtf(wj.x * wj.x + wj.y * wj.y + wj.z * wj.z);
+                wjl = __builtin_sqrtf(dist2_fma(wj.x, wj.y, wj.z));
             }
             return PosLambda{pj, wjl};
         },
         [&](const PosLambda& r) {
             const float4& pj = r.p;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float3 g = grad<kFast>(dx, dy, dz, dx * dx + dy * dy + dz * dz, s);
+            const float3 g = grad<kFast>(dx, dy, dz, dist2_fma(dx, dy, dz), s);
             const float md = r.l - wil;
             ex += md * g.x;
             ey += md * g.y;
             ez += md * g.z;
         });
-    const float el = __builtin_sqrtf(ex * ex + ey * ey + ez * ez);
+    const float el = __builtin_sqrtf(dist2_fma(ex, ey, ez));
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     if (el > 1.0e-6f) {  // normalizationEpsilon, pbf_solver_kernels.cu:367
         nx = ex / el;
         ny = ey / el;
         nz = ez / el;
     }
-    const float fx = strength * (ny * wi.z - nz * wi.y);
-    const float fy = strength * (nz * wi.x - nx * wi.z);
-    const float fz = strength * (nx * wi.y - ny * wi.x);
+    const float fx = strength * cross_fma(ny, wi.z, nz, wi.y);
+    const float fy = strength * cross_fma(nz, wi.x, nx, wi.z);
+    const float fz = strength * cross_fma(nx, wi.y, ny, wi.x);
     out = make_float4(vi.x + dt * fx, vi.y + dt * fy, vi.z + dt * fz, vi.w);
 }
 
@@ -477,7 +477,7 @@ __global__ void k_poly6(const pbf_float3* __restrict__ d, size_t n, float h, flo
     if (i >= n) return;
     const Sph s = sph_consts(h);
     const pbf_float3 q = d[i];
-    out[i] = poly6_d2(q.x * q.x + q.y * q.y + q.z * q.z, s);
+    out[i] = poly6_d2(dist2_fma(q.x, q.y, q.z), s);
 }
 
 __global__ void k_spiky(const pbf_float3* __restrict__ d, size_t n, float h,
@@ -486,7 +486,7 @@ __global__ void k_spiky(const pbf_float3* __restrict__ d, size_t n, float h,
     if (i >= n) return;
     const Sph s = sph_consts(h);
     const pbf_float3 q = d[i];
-    const float3 g = spiky_grad(q.x, q.y, q.z, q.x * q.x + q.y * q.y + q.z * q.z, s);
+    const float3 g = spiky_grad(q.x, q.y, q.z, dist2_fma(q.x, q.y, q.z), s);
     out[i] = pbf_float3{g.x, g.y, g.z};
 }
 
@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(kBlock)
             nbr, stride, i, c, [&](uint32_t j) { return ld4(pin, j); },
             [&](const float4& pj) {
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float d2 = dist2_fma(dx, dy, dz);
                 const float3 gw = grad<true>(dx, dy, dz, d2, s);
                 const float ls = li + pj.w + scorr_value(d2, s.h2, sc);
                 ax += ls * gw.x;
@@ -632,7 +632,7 @@ __global__ void __launch_bounds__(kBlock)
     if (kSkin) {
         const float3 q = container_point(ld4(p0, i), *ci.container, radius);
         const float ex = pi.x - q.x, ey = pi.y - q.y, ez = pi.z - q.z;
-        if (!(ex * ex + ey * ey + ez * ez <= stale2)) atomicOr(status, PBF_STATUS_LIST_STALE);
+        if (!(dist2_fma(ex, ey, ez) <= stale2)) atomicOr(status, PBF_STATUS_LIST_STALE);
     }
     });
 }

@@ -6,8 +6,9 @@
 //    projections are evaluated with FP32 multiplies and adds that are never
 //    contracted into FMAs, so they are bit-identical to a host evaluation of
 //    the same expression (the oracle does exactly that);
-//  * the SPH sums may contract into FMAs; they are compared to the FP64
-//    oracle within a tolerance.
+//  * the SPH sums use FMAs, spelled out wherever an expression holds two
+//    products (dist2_fma, cross_fma) so that every inlined copy of a pass
+//    rounds alike; they are compared to the FP64 oracle within a tolerance.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -90,6 +91,24 @@ __device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// The squared distance of the SPH sums, with its contraction spelled out:
+// fma(dz, dz, fma(dx, dx, dy * dy)).  Written as dx*dx + dy*dy + dz*dz, the
+// compiler may contract the first sum around either product, and it chooses
+// per inlined copy: two forms of one pass (batched and single-scene, the
+// list build's XSPH and the separate pass) would then round differently,
+// and any change elsewhere in a kernel could move its results by an ulp.
+__device__ __forceinline__ float dist2_fma(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+}
+__device__ __forceinline__ double dist2_fma(double dx, double dy, double dz) {
+    return __builtin_fma(dz, dz, __builtin_fma(dx, dx, dy * dy));
+}
+// a * b - c * d with its contraction spelled out (a component of a cross
+// product), for the same reason: fma(a, b, -(c * d)).
+__device__ __forceinline__ float cross_fma(float a, float b, float c, float d) {
+    return __builtin_fmaf(a, b, -(c * d));
+}
+
 // (p* - x) / dt as the reference's velocity update stores it
 // (pbf_solver_kernels.cu:207): rounded, never fused into the XSPH sums that
 // read it back.
@@ -133,15 +152,12 @@ __device__ __forceinline__ float poly6_d2(float d2, const Sph& s) {
 }
 
 // One neighbour's XSPH term (applyXsphViscosity, pbf_solver_kernels.cu:
-// 242-257): c += (v_j - v_i) W(p*_i - p*_j).
-// The squared distance is spelled out as fma(dz, dz, fma(dx, dx, dy * dy)):
-// left to the compiler, dx * dx + dy * dy may be contracted around either
-// product, and it picks differently in different inlined copies, while the
-// list build's XSPH and the separate pass must agree bit for bit.
+// 242-257): c += (v_j - v_i) W(p*_i - p*_j).  The list build's XSPH and the
+// separate pass both call it, and agree bit for bit.
 __device__ __forceinline__ void xsph_term(float3 pi, float3 vi, float3 pj, float3 vj, const Sph& s,
                                           float3& c) {
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+    const float d2 = dist2_fma(dx, dy, dz);
     const float w = poly6_d2(d2, s);
     c.x += (vj.x - vi.x) * w;
     c.y += (vj.y - vi.y) * w;
