@@ -70,12 +70,13 @@ def test_config3_one_substep_parity_with_fp64_oracle(gpu):
     # The per-particle bars as in the one-step parity tests; the norm bar is
     # 1e-5 or, where an FP32 evaluation of the reference's own arithmetic
     # (the FP32 oracle, same input) is farther than that from FP64, that
-    # distance: measured 4.50e-5 for the FP32 oracle and 4.26e-5 for the GPU
+    # distance: measured 4.04e-5 for the FP32 oracle and 1.49e-5 for the GPU
     # (the state is the GPU's own frame 5, so it moves with the GPU's
     # rounding).
     # Contact branches: the FP32 oracle takes the other branch than FP64 at
-    # a few contact particles of this state (measured 2; the GPU 1, one of
-    # the two), each then off by the whole branch; the GPU may flip as many.
+    # a few contact particles (measured 1 on this state, 2 on an earlier
+    # one, where the GPU flipped one of the two), each then off by the whole
+    # branch; the GPU may flip as many.
     f32_miss, _ = _loose_misses(*out["f32"], po, vo, sc)
     _compare_with_contacts(pg, vg, po, vo, sc, "double dam break 8M, frame 5",
                            norm_bar=max(1e-5, f32_err), flips_allowed=int(f32_miss.sum()))
