@@ -2,9 +2,10 @@
 the fused build adds each neighbour's applyXsphViscosity term
 (src/pbf/pbf_solver_kernels.cu:213-264) as it appends the neighbour to the
 list, with the separate pass's arithmetic and in list order, so the step's
-results are bitwise those of the separate pass.  Covered: the staged waves
-and the per-lane sweep of sparse regions (the spray scene), graph replay,
-a batch of scenes, and the adjoint's tape of the XSPH output.
+results are bitwise those of the separate pass.  Covered: the staged waves,
+the per-lane sweep of sparse regions (the spray scene), rows swept from
+global memory (the dense block), graph replay, a batch of scenes, and the
+adjoint's tape of the XSPH output.
 """
 from __future__ import annotations
 
@@ -38,9 +39,9 @@ def _same(a, b):
 
 @pytest.mark.parametrize("graphs", [False, True])
 def test_fused_xsph_equals_separate_pass(gpu, graphs):
-    from test_parity_gpu import block_with_spray
+    from test_parity_gpu import block_with_spray, dense_block
 
-    for sc in (block_with_spray(), scenes.dam_break(1)):
+    for sc in (block_with_spray(), dense_block(), scenes.dam_break(1)):
         fused, separate = _solver(sc, True, graphs), _solver(sc, False, graphs)
         for _ in range(3):
             fused.step()

@@ -199,6 +199,69 @@ def test_step_neighbor_lists_equal_find_neighbors(gpu, oracle64, scene_name):
     print(f"[{scene_name}] {n} particles, {int(oc.sum())} neighbour entries equal", flush=True)
 
 
+def dense_block():
+    """A block at about 25 particles per grid cell (spacing 0.34 h; about
+    105 neighbours each, under the 256 cap): a 64-particle wave's rows of
+    cells then hold more candidates than the list build stages in LDS, so
+    those rows are swept from global memory."""
+    sc = scenes.small_block(20, 20, 20)
+    pos = sc.positions.copy()
+    pos[:, :3] *= np.float32(0.684)
+    return scenes.Scene("dense_block", pos, sc.velocities, sc.minBounds, sc.maxBounds, sc.params)
+
+
+def _widest_staged_row(pos, sc):
+    """The most candidates any 64-particle wave of the working set (two
+    grid rows at most, each under 14 cells wide) gathers from one (dz, dy)
+    row of cells: the list build stages up to 160 of them in LDS."""
+    h = np.float32(sc.params.smoothingRadius)
+    mn = np.asarray(sc.minBounds, np.float32)
+    dims = np.maximum(1, np.ceil((np.asarray(sc.maxBounds, np.float32) - mn) / h)).astype(np.int64)
+    c = np.clip(np.floor((pos[:, :3] - mn) / h).astype(np.int64), 0, dims - 1)
+    key = c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2])
+    counts = np.bincount(key, minlength=int(dims.prod()))
+    widest = 0
+    for w0 in range(0, len(pos), 64):
+        cw = c[w0:w0 + 64]
+        rows = {}
+        for (x, y, z) in cw:
+            lo, hi = rows.get((y, z), (x, x))
+            rows[(y, z)] = (min(lo, x), max(hi, x))
+        if len(rows) > 2 or any(hi - lo >= 14 for lo, hi in rows.values()):
+            continue
+        tot = np.zeros(9, np.int64)
+        for (y, z), (lo, hi) in rows.items():
+            for s in range(9):
+                yy, zz = y + s % 3 - 1, z + s // 3 - 1
+                if 0 <= yy < dims[1] and 0 <= zz < dims[2]:
+                    xs = np.arange(max(lo - 1, 0), min(hi + 1, dims[0] - 1) + 1)
+                    tot[s] += counts[xs + dims[0] * (yy + dims[1] * zz)].sum()
+        widest = max(widest, int(tot.max()))
+    return widest
+
+
+def test_step_neighbor_lists_of_rows_past_the_lds_stage(gpu, oracle64):
+    """On the dense block the list build sweeps rows of more than 160
+    candidates from global memory; its lists still equal findNeighbors'
+    entry by entry."""
+    sc = dense_block()
+    s = _solver_for(sc, gpu.Solver)
+    s.step()
+    pos, cnt, nbr = s.neighborLists()
+    n, h = sc.n, sc.params.smoothingRadius
+    assert _widest_staged_row(pos, sc) > 160
+    osk, osi, ocs, oce, dims, cells = oracle64.grid_build(pos, sc.minBounds, sc.maxBounds, h)
+    assert np.array_equal(osi, np.arange(n)), "the working set is in cell order"
+    on, oc, of = oracle64.find_neighbors(pos, ocs, oce, dims, sc.minBounds, h, n, h, 256, n)
+    assert of == 0
+    assert np.array_equal(cnt, oc)
+    on = on.reshape(256, n)
+    mask = np.arange(256)[:, None] < oc[None, :]
+    assert np.array_equal(nbr[mask], on[mask])
+    print(f"[dense block] {n} particles, mean count {oc.mean():.1f}, widest row "
+          f"{_widest_staged_row(pos, sc)}, lists equal", flush=True)
+
+
 def _contact_mask(p, sc, band=1e-3):
     """Particles within `band` of a contact: a collider surface inflated by the
     particle radius, or a container wall.  The reference's projection and
