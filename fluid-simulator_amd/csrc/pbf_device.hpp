@@ -539,6 +539,9 @@ __device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ 
         constexpr int kStep = kNbrBatch * kDepth;
         const uint4* groups = reinterpret_cast<const uint4*>(nbr) + i;
         const int lastGroup = (c - 1) / 4;
+        // A list's stride (the particle capacity) is far below 2^32, so a
+        // group's offset is one 32 x 32 -> 64-bit multiply.
+        const uint64_t stride32 = static_cast<uint32_t>(stride);
         uint4 g[kDepth];
 #pragma unroll
         for (int d = 0; d < kDepth; ++d)
@@ -558,7 +561,9 @@ __device__ __forceinline__ void for_each_neighbour(const uint32_t* __restrict__ 
             for (int u = 0; u < kStep; ++u) r[u] = load(j[u]);
 #pragma unroll
             for (int d = 0; d < kDepth; ++d)
-                g[d] = groups[static_cast<size_t>(min(q0 / 4 + kDepth + d, lastGroup)) * stride];
+                g[d] = groups[static_cast<uint64_t>(static_cast<uint32_t>(
+                                  min(q0 / 4 + kDepth + d, lastGroup))) *
+                              stride32];
 #pragma unroll
             for (int u = 0; u < kStep; ++u)
                 if (q0 + u < c) use(r[u]);
