@@ -6,13 +6,16 @@
 // stable ids map it back to API order on copy-out.  Per substep the device
 // runs (reference stage -> kernel here):
 //   integrate + v_in snapshot + first cell keys      -> k_predict_key
-//   per iteration: grid build, reorder               -> radix sort, k_ranges_gather
+//   per iteration: grid build, reorder               -> re-sort (or radix sort),
+//                                                         k_ranges_gather
 //                  neighbour lists                    -> k_build_lists (LDS cell sweep)
 //                  lambda (into p*.w)                 -> k_lambda_list
 //                  delta-p, apply, collision solve    -> k_delta_apply_collide
 //                  (restores p*.w from x.w; writes the next build's keys)
 //   post-solve rebuild (if XSPH or vorticity)         -> as above
-//   velocity update + XSPH                            -> k_update_xsph
+//   velocity update + XSPH                            -> in the post-solve
+//                                                         k_build_lists, or
+//                                                         k_update_xsph
 //   vorticity (+ (x, |omega|) records)                -> k_vorticity_list
 //   confinement + velocity contacts                   -> k_confine_resolve
 // The lists hold the reference's neighbours in the reference's order, stored
