@@ -8,6 +8,8 @@
     five frames of the benchmark's three-substep frames checked for finite
     state inside the container, the grid of the GPU's own positions
     bit-exact against the oracle's;
+  * the 1M dam break at one substep per step: one step against the FP64
+    oracle from a state ten steps in, at the same bars;
   * the 256-scene batched workload (about 10.2M particles): one batched
     frame equals, bit for bit, each of a seeded subset of scenes stepped
     alone at its full particle count.
@@ -79,6 +81,43 @@ def test_config3_one_substep_parity_with_fp64_oracle(gpu):
     # branch; the GPU may flip as many.
     f32_miss, _ = _loose_misses(*out["f32"], po, vo, sc)
     _compare_with_contacts(pg, vg, po, vo, sc, "double dam break 8M, frame 5",
+                           norm_bar=max(1e-5, f32_err), flips_allowed=int(f32_miss.sum()))
+
+
+def test_config2_one_substep_parity_with_fp64_oracle(gpu):
+    """The 1M dam break (the headline workload) at one substep per step, as
+    the 8M test: one step from a state ten steps in, against the FP64
+    oracle from the same copied state, at the one-step bars
+    (contact-tolerant, the velocity norm bar at max(1e-5, the FP32 oracle's
+    own distance from FP64))."""
+    from oracle import OracleSolver
+    from test_parity_gpu import _compare_with_contacts, _loose_misses, _solver_for
+
+    sc = scenes.dam_break(2, substeps=1)
+    g = _solver_for(sc, gpu.Solver)
+    g.run(10)
+    sc.positions, sc.velocities = g.copyPositionsToHost(), g.copyVelocitiesToHost()
+    g = _solver_for(sc, gpu.Solver)
+    g.step()
+    pg, vg = g.copyPositionsToHost(), g.copyVelocitiesToHost()
+    del g
+    out = {}
+    for prec in ("f64", "f32"):
+        o = _solver_for(sc, lambda: OracleSolver(prec))
+        o.step()
+        out[prec] = (o.copyPositionsToHost(), o.copyVelocitiesToHost())
+        del o
+    po, vo = out["f64"]
+
+    def norm_rel(v):
+        d = v[:, :3].astype(np.float64) - vo[:, :3]
+        return np.linalg.norm(d) / np.linalg.norm(vo[:, :3].astype(np.float64))
+
+    f32_err = norm_rel(out["f32"][1])
+    print(f"1M step 11 velocity norm error vs FP64: GPU {norm_rel(vg):.3e}, "
+          f"FP32 oracle {f32_err:.3e}", flush=True)
+    f32_miss, _ = _loose_misses(*out["f32"], po, vo, sc)
+    _compare_with_contacts(pg, vg, po, vo, sc, "dam break 1M, step 11",
                            norm_bar=max(1e-5, f32_err), flips_allowed=int(f32_miss.sum()))
 
 
